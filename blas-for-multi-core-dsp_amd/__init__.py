@@ -1,0 +1,154 @@
+"""Python binding of the blasdsp C ABI (include/blasdsp.h) over ctypes.
+
+The product is the C library lib/libblasdsp.so; this module only loads it,
+declares the argument types of every exported function and offers
+tensor-level helpers for tests and the benchmark.  Device memory and streams
+come from PyTorch (plumbing only): a torch tensor's data pointer and the
+current HIP stream are handed to the C functions unchanged.
+
+There is no fallback: if the library is missing or fails to load, every call
+raises.  Build it with `make -C blas-for-multi-core-dsp_amd` (or
+__graft_entry__.build()).
+"""
+from __future__ import annotations
+
+import ctypes
+import pathlib
+import re
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "lib" / "libblasdsp.so"
+HEADER = HERE.parent / "include" / "blasdsp.h"
+
+SUCCESS = 0
+ERR_INVALID = -22
+ERR_NOMEM = -12
+ERR_DEVICE = -5
+ERR_UNSUPPORTED = -95
+
+ALGO_AUTO, ALGO_TILE, ALGO_SKINNY = 0, 1, 2
+
+UNIFORM, NORMAL, LOGUNIF, PATTERN11, PATTERN7, ONES = range(6)
+
+_i64 = ctypes.c_int64
+_vp = ctypes.c_void_p
+_d = ctypes.c_double
+_f = ctypes.c_float
+_lib = None
+
+
+class BlasdspError(RuntimeError):
+    def __init__(self, fn: str, status: int):
+        super().__init__(f"{fn} returned {status}")
+        self.status = status
+
+
+def header_functions() -> list[str]:
+    """Names of every function declared in include/blasdsp.h."""
+    text = HEADER.read_text()
+    names = re.findall(r"^BLASDSP_API\s+[\w\s\*]*?\b(\w+)\s*\(", text, flags=re.M)
+    names += re.findall(r"^BLASDSP_DECLARE_IRR\((\w+),", text, flags=re.M)
+    return names
+
+
+def _declare(L: ctypes.CDLL) -> None:
+    mode_args = [_i64, _i64, _i64, _i64, _i64, _i64, _vp, _vp, _vp]
+    for name in header_functions():
+        fn = getattr(L, name)
+        fn.restype = ctypes.c_int
+    L.blasdsp_version.restype = ctypes.c_char_p
+    L.blasdsp_status_string.restype = ctypes.c_char_p
+    L.blasdsp_status_string.argtypes = [ctypes.c_int]
+    gen = [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64]
+    L.blasdsp_dgemm.argtypes = gen + [_d, _vp, _i64, _vp, _i64, _d, _vp, _i64, _vp]
+    L.blasdsp_sgemm.argtypes = gen + [_f, _vp, _i64, _vp, _i64, _f, _vp, _i64, _vp]
+    L.blasdsp_dgemm_ex.argtypes = gen + [_d, _vp, _i64, _vp, _i64, _d, _vp, _i64, ctypes.c_int,
+                                         ctypes.c_int, _vp]
+    L.blasdsp_sgemm_ex.argtypes = gen + [_f, _vp, _i64, _vp, _i64, _f, _vp, _i64, ctypes.c_int,
+                                         ctypes.c_int, _vp]
+    for name in header_functions():
+        if re.match(r"^(dgemm|irr_\w+_dgemm)_(NN|NT|TN|TT|null)$", name):
+            getattr(L, name).argtypes = mode_args + [_d, _d, _vp]
+        elif re.match(r"^(sgemm|irr_\w+_sgemm)_(NN|NT|TN|TT|null)$", name):
+            getattr(L, name).argtypes = mode_args + [_f, _f, _vp]
+    L.blasdsp_fill_f64.argtypes = [_vp, _i64, _i64, _i64, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _d, _vp]
+    L.blasdsp_fill_f32.argtypes = [_vp, _i64, _i64, _i64, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _f, _vp]
+    for extra in _EXTRA_DECLS:
+        extra(L)
+
+
+_EXTRA_DECLS: list = []
+
+
+def lib() -> ctypes.CDLL:
+    """Load lib/libblasdsp.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise FileNotFoundError(f"{LIB_PATH} is not built; run make -C {HERE}")
+        L = ctypes.CDLL(str(LIB_PATH))
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def check(fn: str, status: int) -> None:
+    if status != SUCCESS:
+        raise BlasdspError(fn, status)
+
+
+# ---- tensor helpers ---------------------------------------------------------
+
+def _stream(stream):
+    import torch
+
+    if stream is None:
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if isinstance(stream, int):
+        return ctypes.c_void_p(stream)
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def _ld(t) -> int:
+    if t.dim() != 2 or (t.stride(1) != 1 and t.shape[1] > 1):
+        raise ValueError("matrices must be 2-D with contiguous rows")
+    return int(t.stride(0)) if t.shape[0] > 1 else max(int(t.shape[1]), 1)
+
+
+def gemm(transA: str, transB: str, A, B, C, alpha: float = 1.0, beta: float = 0.0,
+         algo: int = ALGO_AUTO, splitk: int = 0, stream=None, raise_on_error: bool = True) -> int:
+    """C = alpha op(A) op(B) + beta C on torch device tensors (in place on C).
+
+    A, B, C are the STORED matrices (A is K x M for transA == 'T').  Leading
+    dimensions are the tensors' row strides, so column slices of wider
+    buffers express padded leading dimensions."""
+    import torch
+
+    M, N = C.shape
+    K = A.shape[0] if transA.upper() == "T" else A.shape[1]
+    if C.dtype == torch.float64:
+        fn, name = lib().blasdsp_dgemm_ex, "blasdsp_dgemm_ex"
+    elif C.dtype == torch.float32:
+        fn, name = lib().blasdsp_sgemm_ex, "blasdsp_sgemm_ex"
+    else:
+        raise TypeError(f"unsupported dtype {C.dtype}")
+    assert A.dtype == B.dtype == C.dtype
+    st = fn(transA.encode(), transB.encode(), M, N, K, alpha, A.data_ptr(), _ld(A), B.data_ptr(), _ld(B),
+            beta, C.data_ptr(), _ld(C), algo, splitk, _stream(stream))
+    if raise_on_error:
+        check(name, st)
+    return st
+
+
+def fill(X, rows: int, cols: int, dist: int, seed: int, trans: bool = False, pad: float = 0.0, stream=None):
+    """Device generator into the storage tensor X (rows x cols logical)."""
+    import torch
+
+    fn = lib().blasdsp_fill_f64 if X.dtype == torch.float64 else lib().blasdsp_fill_f32
+    check("blasdsp_fill", fn(X.data_ptr(), rows, cols, int(X.stride(0)), int(trans), dist, seed, pad,
+                             _stream(stream)))
+    return X
+
+
+def version() -> str:
+    return lib().blasdsp_version().decode()

@@ -1,0 +1,294 @@
+// blasdsp_api.hip — the C ABI: argument checks, kernel selection, workspace.
+//
+// Replaces the reference's launcher shims (launch_dgemm_NN & co.,
+// dgemm_release_V1.1/device_code/dgemm_launcher.c:10-28,
+// sgemm_release_V1.3.2/device_code/sgemm_launcher.c:11-37) and the
+// hthread_group_exec-by-name boundary (dgemm_release_V1.1/host_code/
+// dgemm_demo.c:151): the host calls a C function, which validates, picks a
+// kernel and enqueues it on the caller's HIP stream.  Unlike the reference
+// (void kernels, silent wrong results on unsupported shapes,
+// dgemm_release_V1.1/readme.txt:11-15) every shape is supported and every
+// call returns a status.
+#include <errno.h>
+#include <stdlib.h>
+
+#include <map>
+#include <mutex>
+#include <utility>
+
+#include "../../include/blasdsp.h"
+#include "internal.h"
+
+namespace blasdsp {
+namespace {
+
+thread_local int g_last_hip_error = 0;
+
+int hip_status(hipError_t e)
+{
+    if (e == hipSuccess) return BLASDSP_SUCCESS;
+    g_last_hip_error = (int)e;
+    if (e == hipErrorOutOfMemory) return BLASDSP_ERR_NOMEM;
+    if (e == hipErrorInvalidValue) return BLASDSP_ERR_INVALID;
+    return BLASDSP_ERR_DEVICE;
+}
+
+// Split-K workspace, one grow-only buffer per (device, stream): calls on one
+// stream are ordered, so reuse within a stream is safe; distinct streams never
+// share a buffer.  Growing frees the old buffer with hipFree, which waits for
+// the device, so in-flight users of the old buffer finish first.
+struct Workspace {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+std::mutex g_ws_mutex;
+std::map<std::pair<int, hipStream_t>, Workspace> g_ws;
+
+hipError_t workspace(hipStream_t s, size_t bytes, void **out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    Workspace &w = g_ws[std::make_pair(dev, s)];
+    if (w.bytes < bytes) {
+        if (w.ptr) {
+            e = hipFree(w.ptr);
+            w.ptr = nullptr;
+            w.bytes = 0;
+            if (e != hipSuccess) return e;
+        }
+        size_t want = bytes + bytes / 4;  // headroom against creeping growth
+        e = hipMalloc(&w.ptr, want);
+        if (e != hipSuccess) {
+            w.ptr = nullptr;
+            return e;
+        }
+        w.bytes = want;
+    }
+    *out = w.ptr;
+    return hipSuccess;
+}
+
+int num_cus()
+{
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (cached[dev] == 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+            n = 256;
+        cached[dev] = n;
+    }
+    return cached[dev];
+}
+
+bool parse_trans(char t, bool &out)
+{
+    switch (t) {
+    case 'N': case 'n': out = false; return true;
+    case 'T': case 't': case 'C': case 'c': out = true; return true;
+    default: return false;
+    }
+}
+
+// Argument checks shared by every entry point (reference shape limits,
+// dgemm_release_V1.1/readme.txt:11-15, are not carried over: any M, N, K >= 0
+// and any leading dimension covering a row are accepted).
+template <typename T>
+int check(Gemm<T> &g, char transA, char transB, int64_t M, int64_t N, int64_t K, T alpha,
+          const T *A, int64_t lda, const T *B, int64_t ldb, T beta, T *C, int64_t ldc)
+{
+    if (!parse_trans(transA, g.ta) || !parse_trans(transB, g.tb)) return BLASDSP_ERR_INVALID;
+    if (M < 0 || N < 0 || K < 0) return BLASDSP_ERR_INVALID;
+    const int64_t a_cols = g.ta ? M : K, b_cols = g.tb ? K : N;
+    if (lda < (a_cols > 1 ? a_cols : 1)) return BLASDSP_ERR_INVALID;
+    if (ldb < (b_cols > 1 ? b_cols : 1)) return BLASDSP_ERR_INVALID;
+    if (ldc < (N > 1 ? N : 1)) return BLASDSP_ERR_INVALID;
+    if (M > 0 && N > 0) {
+        if (!C) return BLASDSP_ERR_INVALID;
+        if (K > 0 && alpha != T(0) && (!A || !B)) return BLASDSP_ERR_INVALID;
+    }
+    g.M = M; g.N = N; g.K = K;
+    g.alpha = alpha; g.beta = beta;
+    g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
+    return BLASDSP_SUCCESS;
+}
+
+template <typename T>
+bool skinny_shape(const Gemm<T> &g)
+{
+    const int64_t small = g.M <= g.N ? g.M : g.N;
+    const int64_t large = g.M <= g.N ? g.N : g.M;
+    return small <= 64 && large >= 2 * small;
+}
+
+template <typename T>
+int run(const Gemm<T> &g, int algo, int splitk, hipStream_t s)
+{
+    if (g.M == 0 || g.N == 0) return BLASDSP_SUCCESS;
+    if (g.K == 0 || g.alpha == T(0)) return hip_status(launch_scale(g, s));
+    const int64_t small = g.M <= g.N ? g.M : g.N;
+    bool use_skinny;
+    if (algo == BLASDSP_ALGO_TILE) use_skinny = false;
+    else if (algo == BLASDSP_ALGO_SKINNY) use_skinny = small <= 64;  // larger: tile kernel
+    else if (algo == BLASDSP_ALGO_AUTO) use_skinny = skinny_shape(g);
+    else return BLASDSP_ERR_INVALID;
+    if (!use_skinny) return hip_status(launch_tile(g, s));
+
+    if (splitk <= 0) splitk = skinny_pick_splitk(g, num_cus());
+    if (splitk > 256) return BLASDSP_ERR_INVALID;
+    T *work = nullptr;
+    if (splitk > 1) {
+        const size_t bytes = (size_t)splitk * (size_t)g.M * (size_t)g.N * sizeof(T);
+        void *p = nullptr;
+        hipError_t e = workspace(s, bytes, &p);
+        if (e != hipSuccess) return hip_status(e);
+        work = (T *)p;
+    }
+    return hip_status(launch_skinny(g, splitk, work, s));
+}
+
+template <typename T>
+int gemm_entry(char ta, char tb, int64_t M, int64_t N, int64_t K, T alpha, const T *A, int64_t lda,
+               const T *B, int64_t ldb, T beta, T *C, int64_t ldc, int algo, int splitk, void *stream)
+{
+    Gemm<T> g;
+    int st = check<T>(g, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+    if (st != BLASDSP_SUCCESS) return st;
+    return run(g, algo, splitk, (hipStream_t)stream);
+}
+
+template <typename T>
+int null_entry(char ta, char tb, int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N,
+               int64_t K, const T *A, const T *B, T *C, T alpha, T beta, void *stream)
+{
+    Gemm<T> g;
+    int st = check<T>(g, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+    if (st != BLASDSP_SUCCESS) return st;
+    return hip_status(launch_null((hipStream_t)stream));
+}
+
+}  // namespace
+}  // namespace blasdsp
+
+using blasdsp::gemm_entry;
+
+extern "C" {
+
+const char *blasdsp_version(void) { return "blasdsp 0.1.0 (gfx950)"; }
+
+const char *blasdsp_status_string(int status)
+{
+    switch (status) {
+    case BLASDSP_SUCCESS: return "success";
+    case BLASDSP_ERR_INVALID: return "invalid argument";
+    case BLASDSP_ERR_NOMEM: return "out of memory";
+    case BLASDSP_ERR_DEVICE: return "HIP runtime error";
+    case BLASDSP_ERR_UNSUPPORTED: return "unsupported";
+    default: return "unknown status";
+    }
+}
+
+int blasdsp_last_hip_error(void) { return blasdsp::g_last_hip_error; }
+
+int blasdsp_dgemm(char transA, char transB, int64_t M, int64_t N, int64_t K, double alpha,
+                  const double *A, int64_t lda, const double *B, int64_t ldb, double beta, double *C,
+                  int64_t ldc, void *stream)
+{
+    return gemm_entry<double>(transA, transB, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                              BLASDSP_ALGO_AUTO, 0, stream);
+}
+
+int blasdsp_sgemm(char transA, char transB, int64_t M, int64_t N, int64_t K, float alpha,
+                  const float *A, int64_t lda, const float *B, int64_t ldb, float beta, float *C,
+                  int64_t ldc, void *stream)
+{
+    return gemm_entry<float>(transA, transB, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                             BLASDSP_ALGO_AUTO, 0, stream);
+}
+
+int blasdsp_dgemm_ex(char transA, char transB, int64_t M, int64_t N, int64_t K, double alpha,
+                     const double *A, int64_t lda, const double *B, int64_t ldb, double beta,
+                     double *C, int64_t ldc, int algo, int splitk, void *stream)
+{
+    return gemm_entry<double>(transA, transB, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, algo,
+                              splitk, stream);
+}
+
+int blasdsp_sgemm_ex(char transA, char transB, int64_t M, int64_t N, int64_t K, float alpha,
+                     const float *A, int64_t lda, const float *B, int64_t ldb, float beta, float *C,
+                     int64_t ldc, int algo, int splitk, void *stream)
+{
+    return gemm_entry<float>(transA, transB, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, algo,
+                             splitk, stream);
+}
+
+#define BLASDSP_MODE_ENTRY(name, T, ta, tb, algo)                                                 \
+    int name(int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N, int64_t K, const T *A,  \
+             const T *B, T *C, T alpha, T beta, void *stream)                                    \
+    {                                                                                             \
+        return gemm_entry<T>(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, algo, 0,       \
+                             stream);                                                             \
+    }
+
+BLASDSP_MODE_ENTRY(dgemm_NN, double, 'N', 'N', BLASDSP_ALGO_AUTO)
+BLASDSP_MODE_ENTRY(dgemm_NT, double, 'N', 'T', BLASDSP_ALGO_AUTO)
+BLASDSP_MODE_ENTRY(dgemm_TN, double, 'T', 'N', BLASDSP_ALGO_AUTO)
+BLASDSP_MODE_ENTRY(dgemm_TT, double, 'T', 'T', BLASDSP_ALGO_AUTO)
+BLASDSP_MODE_ENTRY(sgemm_NN, float, 'N', 'N', BLASDSP_ALGO_AUTO)
+BLASDSP_MODE_ENTRY(sgemm_NT, float, 'N', 'T', BLASDSP_ALGO_AUTO)
+BLASDSP_MODE_ENTRY(sgemm_TN, float, 'T', 'N', BLASDSP_ALGO_AUTO)
+BLASDSP_MODE_ENTRY(sgemm_TT, float, 'T', 'T', BLASDSP_ALGO_AUTO)
+BLASDSP_MODE_ENTRY(irr_shortA_dgemm_NN, double, 'N', 'N', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(irr_shortA_dgemm_NT, double, 'N', 'T', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(irr_shortA_dgemm_TN, double, 'T', 'N', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(irr_shortA_dgemm_TT, double, 'T', 'T', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(irr_shortA_sgemm_NN, float, 'N', 'N', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(irr_shortA_sgemm_NT, float, 'N', 'T', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(irr_shortA_sgemm_TN, float, 'T', 'N', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(irr_shortA_sgemm_TT, float, 'T', 'T', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(irr_thinB_dgemm_NN, double, 'N', 'N', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(irr_thinB_dgemm_NT, double, 'N', 'T', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(irr_thinB_dgemm_TN, double, 'T', 'N', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(irr_thinB_dgemm_TT, double, 'T', 'T', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(irr_thinB_sgemm_NN, float, 'N', 'N', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(irr_thinB_sgemm_NT, float, 'N', 'T', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(irr_thinB_sgemm_TN, float, 'T', 'N', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(irr_thinB_sgemm_TT, float, 'T', 'T', BLASDSP_ALGO_SKINNY)
+#undef BLASDSP_MODE_ENTRY
+
+int dgemm_null(int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N, int64_t K,
+               const double *A, const double *B, double *C, double alpha, double beta, void *stream)
+{
+    return blasdsp::null_entry<double>('N', 'N', lda, ldb, ldc, M, N, K, A, B, C, alpha, beta, stream);
+}
+
+int sgemm_null(int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N, int64_t K,
+               const float *A, const float *B, float *C, float alpha, float beta, void *stream)
+{
+    return blasdsp::null_entry<float>('N', 'N', lda, ldb, ldc, M, N, K, A, B, C, alpha, beta, stream);
+}
+
+int blasdsp_fill_f64(double *X, int64_t rows, int64_t cols, int64_t ld, int trans, int dist,
+                     uint64_t seed, double pad, void *stream)
+{
+    if (rows < 0 || cols < 0 || dist < 0 || dist > 5) return BLASDSP_ERR_INVALID;
+    if (ld < ((trans ? rows : cols) > 1 ? (trans ? rows : cols) : 1)) return BLASDSP_ERR_INVALID;
+    if (rows > 0 && cols > 0 && !X) return BLASDSP_ERR_INVALID;
+    return blasdsp::hip_status(
+        blasdsp::launch_fill<double>(X, rows, cols, ld, trans, dist, seed, pad, (hipStream_t)stream));
+}
+
+int blasdsp_fill_f32(float *X, int64_t rows, int64_t cols, int64_t ld, int trans, int dist,
+                     uint64_t seed, float pad, void *stream)
+{
+    if (rows < 0 || cols < 0 || dist < 0 || dist > 5) return BLASDSP_ERR_INVALID;
+    if (ld < ((trans ? rows : cols) > 1 ? (trans ? rows : cols) : 1)) return BLASDSP_ERR_INVALID;
+    if (rows > 0 && cols > 0 && !X) return BLASDSP_ERR_INVALID;
+    return blasdsp::hip_status(
+        blasdsp::launch_fill<float>(X, rows, cols, ld, trans, dist, seed, pad, (hipStream_t)stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,395 @@
+// gemm_skinny.hip — bandwidth-bound GEMM for the irregular shapes: shortA
+// (M small, op(B) K x N streamed) and thinB (N small, op(A) M x K streamed).
+//
+// What it replaces.  irr_shortA_{d,s}gemm_* keep the small A whole on chip and
+// split N across DSP cores; irr_thinB_{d,s}gemm_* keep B whole, split M, and
+// compute C^T through "transpose + shortA + transpose"
+// (irr_shortA_dgemm_release_V0.0/device_code/irr_shortA_dgemm_kernel.c:1607-2677,
+// irr_thinB_sgemm_release_V1.2/device_code/irr_thinB_sgemm_kernel.c:2910-5157).
+//
+// One kernel serves both.  It computes O = S * L where S ("small", Ms x K,
+// Ms <= 64) is the small operand and L ("large", K x Nl) is the streamed one:
+//   shortA:  S = op(A), L = op(B), O = C           (stored C[ms][nl])
+//   thinB:   S = op(B)^T, L = op(A)^T, O = C^T     (stored C[nl][ms])
+// so thinB is shortA with the operand roles swapped and a transposed store —
+// the same identity the reference uses, with the transposes absorbed into the
+// index maps instead of executed.
+//
+// Each wave owns a 64-column panel of O (all Ms rows, MT = Ms/16 MFMA row
+// tiles x 4 column tiles) and walks K in blocks of 16.  L is read straight
+// from HBM into MFMA operand registers with 16-byte vector loads and no LDS
+// round trip: every element of L is loaded exactly once.  The MFMA K index is
+// permuted so that each lane's vector holds whole operands:
+//   * L stored k-major ([k][nl], nl contiguous): lane (c, g) = (l%16, l/16)
+//     loads row k0+4g+j, columns n0 + V*c .. + V-1 at virtual step j; element
+//     e of the vector is the B operand of column tile e, which therefore
+//     covers columns n0 + V*c + e (an interleaved column map undone at the
+//     store);
+//   * L stored nl-major ([nl][k], k contiguous): lane (c, g) loads the 4-run
+//     k0+4g .. k0+4g+3 of row n0+16t+c, one run per column tile t.
+// In both cases virtual step j pairs lane group g with k = k0 + 4g + j, and S
+// is read with the same pairing (a 4-run along k when S is k-contiguous).
+// S is tiny (<= 64 x K) and is served from L1/L2.
+//
+// When the Nl panels alone do not give >= 4 workgroups per CU, K is split
+// into slices (grid.y); each slice writes its partial O to a workspace and a
+// second kernel sums the slices in fixed order (deterministic, no atomics) and
+// applies C = alpha * sum + beta * C.
+#include "internal.h"
+
+namespace blasdsp {
+namespace {
+
+template <typename T>
+struct SK;
+
+template <>
+struct SK<double> {
+    typedef double acc_t __attribute__((ext_vector_type(4)));
+    typedef double vec_t __attribute__((ext_vector_type(2)));
+    static constexpr int V = 2;
+    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+
+template <>
+struct SK<float> {
+    typedef float acc_t __attribute__((ext_vector_type(4)));
+    typedef float vec_t __attribute__((ext_vector_type(4)));
+    static constexpr int V = 4;
+    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+template <typename T>
+struct SkinnyArgs {
+    const T *S;
+    int64_t lds;
+    const T *L;
+    int64_t ldl;
+    T *C;
+    int64_t ldc;
+    T *W;             // split partials [split][Ms][Nl], used when splitk > 1
+    int64_t Ms, Nl, K;
+    int64_t kslice;   // K per slice, a multiple of 16
+    int splitk;
+    T alpha, beta;
+    bool l_vec, s_vec;
+};
+
+constexpr int kWaves = 4;      // waves per workgroup, each an independent panel
+constexpr int kPanel = 64;     // O columns per wave
+constexpr int kBlockK = 16;    // K per step
+
+// Column of O held by lane column c of MFMA column tile t.
+template <typename T, bool L_KC>
+__device__ __forceinline__ int col_of(int t, int c)
+{
+    constexpr int V = SK<T>::V;
+    if (L_KC) return 16 * t + c;
+    // t = h * V + e: vector half h, element e
+    return (t / V) * 16 * V + V * c + (t % V);
+}
+
+// Registers for one 16-deep K block of one wave.
+template <typename T, int MT, bool S_KC, bool L_KC>
+struct Block {
+    using vec_t = typename SK<T>::vec_t;
+    static constexpr int V = SK<T>::V;
+    static constexpr int NLV = 16 / V;   // 16-byte vectors of L per lane per block (4 or 8)
+    static constexpr int NSV = S_KC ? 4 / V : 0;
+    vec_t l[NLV];
+    // S: k-contiguous -> 4-runs as vectors; m-contiguous -> scalars per step
+    vec_t s_run[MT][S_KC ? (4 / V) : 1];
+    T s_scl[MT][S_KC ? 1 : 4];
+
+    // L operand of column tile t at virtual step j.
+    __device__ __forceinline__ T lop(int t, int j) const
+    {
+        if (L_KC) {
+            const int idx = t * (4 / V) + j / V;   // t's 4-run, vector j/V
+            return l[idx][j % V];
+        }
+        // nl-contiguous: vector (j, h) with h = t / V, element e = t % V
+        return l[j * (4 / V) + t / V][t % V];
+    }
+    __device__ __forceinline__ T sop(int mt, int j) const
+    {
+        if (S_KC) return s_run[mt][j / V][j % V];
+        return s_scl[mt][j];
+    }
+};
+
+template <typename T, int MT, bool S_KC, bool L_KC, bool MASK>
+__device__ __forceinline__ void load_block(Block<T, MT, S_KC, L_KC> &b, const SkinnyArgs<T> &a,
+                                           int64_t n0, int64_t k0, int64_t kend, int c, int g)
+{
+    using vec_t = typename SK<T>::vec_t;
+    constexpr int V = SK<T>::V;
+    // ---- L ----
+    if (L_KC) {
+        // t-th column tile: row n0 + 16t + c, k-run k0 + 4g .. +3 in 4/V vectors
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+#pragma unroll
+            for (int q = 0; q < 4 / V; ++q) {
+                const int64_t row = n0 + 16 * t + c;
+                const int64_t kk = k0 + 4 * g + q * V;
+                vec_t v;
+                if (!MASK) {
+                    v = *reinterpret_cast<const vec_t *>(a.L + row * a.ldl + kk);
+                } else if (a.l_vec && row < a.Nl && kk + V <= kend) {
+                    v = *reinterpret_cast<const vec_t *>(a.L + row * a.ldl + kk);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < V; ++e)
+                        v[e] = (row < a.Nl && kk + e < kend) ? a.L[row * a.ldl + kk + e] : T(0);
+                }
+                b.l[t * (4 / V) + q] = v;
+            }
+        }
+    } else {
+        // step j: row k0 + 4g + j, columns n0 + h*16V + V*c .. + V-1
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+            for (int h = 0; h < 4 / V; ++h) {
+                const int64_t kk = k0 + 4 * g + j;
+                const int64_t col = n0 + h * 16 * V + V * c;
+                vec_t v;
+                if (!MASK) {
+                    v = *reinterpret_cast<const vec_t *>(a.L + kk * a.ldl + col);
+                } else if (a.l_vec && kk < kend && col + V <= a.Nl) {
+                    v = *reinterpret_cast<const vec_t *>(a.L + kk * a.ldl + col);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < V; ++e)
+                        v[e] = (kk < kend && col + e < a.Nl) ? a.L[kk * a.ldl + col + e] : T(0);
+                }
+                b.l[j * (4 / V) + h] = v;
+            }
+        }
+    }
+    // ---- S (always bounds-checked on Ms; cheap, L1/L2 resident) ----
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+        const int64_t ms = mt * 16 + c;
+        if (S_KC) {
+#pragma unroll
+            for (int q = 0; q < 4 / V; ++q) {
+                const int64_t kk = k0 + 4 * g + q * V;
+                vec_t v;
+                if (ms < a.Ms && a.s_vec && (!MASK || kk + V <= kend)) {
+                    v = *reinterpret_cast<const vec_t *>(a.S + ms * a.lds + kk);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < V; ++e)
+                        v[e] = (ms < a.Ms && kk + e < kend) ? a.S[ms * a.lds + kk + e] : T(0);
+                }
+                b.s_run[mt][q] = v;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int64_t kk = k0 + 4 * g + j;
+                b.s_scl[mt][j] = (ms < a.Ms && (!MASK || kk < kend)) ? a.S[kk * a.lds + ms] : T(0);
+            }
+        }
+    }
+}
+
+template <typename T, int MT, bool S_KC, bool L_KC>
+__device__ __forceinline__ void mma_block(typename SK<T>::acc_t (&acc)[MT][4],
+                                          const Block<T, MT, S_KC, L_KC> &b)
+{
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[mt][t] = SK<T>::mma(b.sop(mt, j), b.lop(t, j), acc[mt][t]);
+}
+
+template <typename T, int MT, bool S_KC, bool L_KC, bool TRANS_OUT>
+__global__ void __launch_bounds__(256) skinny_kernel(SkinnyArgs<T> a)
+{
+    using acc_t = typename SK<T>::acc_t;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c = lane & 15, g = lane >> 4;
+    const int64_t n0 = ((int64_t)blockIdx.x * kWaves + wave) * kPanel;
+    if (n0 >= a.Nl) return;  // no barriers in this kernel: idle waves may leave
+    const int split = blockIdx.y;
+    const int64_t kb = (int64_t)split * a.kslice;
+    const int64_t ke = min(a.K, kb + a.kslice);
+
+    acc_t acc[MT][4];
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[i][t] = acc_t{0, 0, 0, 0};
+
+    const bool panel_full = a.l_vec && (L_KC ? (n0 + kPanel <= a.Nl) : (n0 + kPanel <= a.Nl));
+    const int64_t kfull = panel_full ? kb + ((ke - kb) / kBlockK) * kBlockK : kb;
+
+    // Full blocks: unmasked L loads, register double buffer.
+    int64_t k0 = kb;
+    if (k0 < kfull) {
+        Block<T, MT, S_KC, L_KC> cur, nxt;
+        load_block<T, MT, S_KC, L_KC, false>(cur, a, n0, k0, ke, c, g);
+        for (; k0 + kBlockK < kfull; k0 += kBlockK) {
+            load_block<T, MT, S_KC, L_KC, false>(nxt, a, n0, k0 + kBlockK, ke, c, g);
+            mma_block<T, MT, S_KC, L_KC>(acc, cur);
+            cur = nxt;
+        }
+        mma_block<T, MT, S_KC, L_KC>(acc, cur);
+        k0 = kfull;
+    }
+    // Ragged tail (partial panel or partial K block): masked loads.
+    for (; k0 < ke; k0 += kBlockK) {
+        Block<T, MT, S_KC, L_KC> b;
+        load_block<T, MT, S_KC, L_KC, true>(b, a, n0, k0, ke, c, g);
+        mma_block<T, MT, S_KC, L_KC>(acc, b);
+    }
+
+    // Store: lane (c, g), register r of tile (mt, t) is O[mt*16 + 4g + r][n0 + col_of(t, c)].
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int64_t nl = n0 + col_of<T, L_KC>(t, c);
+            if (nl >= a.Nl) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t ms = mt * 16 + 4 * g + r;
+                if (ms >= a.Ms) continue;
+                const T v = acc[mt][t][r];
+                if (a.splitk > 1) {
+                    a.W[((int64_t)split * a.Ms + ms) * a.Nl + nl] = v;
+                } else {
+                    T *p = TRANS_OUT ? a.C + nl * a.ldc + ms : a.C + ms * a.ldc + nl;
+                    T o = a.alpha * v;
+                    if (a.beta != T(0)) o += a.beta * *p;
+                    *p = o;
+                }
+            }
+        }
+    }
+}
+
+// Sum the K slices in order s = 0, 1, ... and apply the epilogue.
+template <typename T, bool TRANS_OUT>
+__global__ void skinny_reduce_kernel(SkinnyArgs<T> a)
+{
+    const int64_t total = a.Ms * a.Nl;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        // TRANS_OUT walks C row-major (nl outer) so the stores coalesce.
+        const int64_t ms = TRANS_OUT ? e % a.Ms : e / a.Nl;
+        const int64_t nl = TRANS_OUT ? e / a.Ms : e % a.Nl;
+        T sum = T(0);
+        for (int s = 0; s < a.splitk; ++s) sum += a.W[((int64_t)s * a.Ms + ms) * a.Nl + nl];
+        T *p = TRANS_OUT ? a.C + nl * a.ldc + ms : a.C + ms * a.ldc + nl;
+        T o = a.alpha * sum;
+        if (a.beta != T(0)) o += a.beta * *p;
+        *p = o;
+    }
+}
+
+template <typename T, int MT, bool S_KC, bool L_KC, bool TRANS_OUT>
+hipError_t launch4(const SkinnyArgs<T> &a, hipStream_t s)
+{
+    const int64_t wgs = (a.Nl + kWaves * kPanel - 1) / (kWaves * kPanel);
+    hipLaunchKernelGGL((skinny_kernel<T, MT, S_KC, L_KC, TRANS_OUT>), dim3((unsigned)wgs, a.splitk),
+                       dim3(kWaves * 64), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || a.splitk == 1) return e;
+    const int64_t total = a.Ms * a.Nl;
+    int64_t blocks = (total + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL((skinny_reduce_kernel<T, TRANS_OUT>), dim3((unsigned)blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+template <typename T, int MT>
+hipError_t launch_mt(const SkinnyArgs<T> &a, bool s_kc, bool l_kc, bool trans_out, hipStream_t s)
+{
+    if (s_kc) {
+        if (l_kc) return trans_out ? launch4<T, MT, true, true, true>(a, s) : launch4<T, MT, true, true, false>(a, s);
+        return trans_out ? launch4<T, MT, true, false, true>(a, s) : launch4<T, MT, true, false, false>(a, s);
+    }
+    if (l_kc) return trans_out ? launch4<T, MT, false, true, true>(a, s) : launch4<T, MT, false, true, false>(a, s);
+    return trans_out ? launch4<T, MT, false, false, true>(a, s) : launch4<T, MT, false, false, false>(a, s);
+}
+
+// Operand roles for a validated GEMM.  shortA when M <= N, thinB otherwise.
+template <typename T>
+void roles(const Gemm<T> &g, SkinnyArgs<T> &a, bool &s_kc, bool &l_kc, bool &trans_out)
+{
+    constexpr int V = SK<T>::V;
+    if (g.M <= g.N) {  // shortA: S = op(A), L = op(B)
+        a.S = g.A; a.lds = g.lda; a.L = g.B; a.ldl = g.ldb;
+        a.Ms = g.M; a.Nl = g.N;
+        s_kc = !g.ta;    // A stored M x K (k contiguous) unless transposed
+        l_kc = g.tb;     // B stored N x K (k contiguous) when transposed
+        trans_out = false;
+    } else {           // thinB: S = op(B)^T, L = op(A)^T, O = C^T
+        a.S = g.B; a.lds = g.ldb; a.L = g.A; a.ldl = g.lda;
+        a.Ms = g.N; a.Nl = g.M;
+        s_kc = g.tb;     // op(B)^T[n][k] = B[n][k] when B is stored N x K
+        l_kc = !g.ta;    // op(A)^T[k][m] = A[m][k]: A stored M x K is m-major rows of k
+        trans_out = true;
+    }
+    a.K = g.K;
+    a.C = g.C; a.ldc = g.ldc;
+    a.alpha = g.alpha; a.beta = g.beta;
+    a.l_vec = ((uintptr_t)a.L % 16 == 0) && (a.ldl % V == 0);
+    a.s_vec = ((uintptr_t)a.S % 16 == 0) && (a.lds % V == 0);
+}
+
+}  // namespace
+
+template <typename T>
+int skinny_pick_splitk(const Gemm<T> &g, int num_cus)
+{
+    const int64_t nl = g.M <= g.N ? g.N : g.M;
+    const int64_t wgs = (nl + kWaves * kPanel - 1) / (kWaves * kPanel);
+    const int64_t want = 4LL * (num_cus > 0 ? num_cus : 256);
+    int64_t sk = (want + wgs - 1) / wgs;
+    const int64_t max_by_k = g.K / 512 > 0 ? g.K / 512 : 1;  // slices of >= 512
+    if (sk > max_by_k) sk = max_by_k;
+    if (sk > 64) sk = 64;
+    if (sk < 1) sk = 1;
+    return (int)sk;
+}
+
+template <typename T>
+hipError_t launch_skinny(const Gemm<T> &g, int splitk, T *work, hipStream_t s)
+{
+    SkinnyArgs<T> a{};
+    bool s_kc, l_kc, trans_out;
+    roles(g, a, s_kc, l_kc, trans_out);
+    if (a.Ms > 64 || a.Ms <= 0 || a.Nl <= 0) return hipErrorInvalidValue;
+    if (splitk < 1) splitk = 1;
+    int64_t kslice = (g.K + splitk - 1) / splitk;
+    kslice = (kslice + kBlockK - 1) / kBlockK * kBlockK;
+    if (kslice < kBlockK) kslice = kBlockK;
+    const int64_t nsplit = (g.K + kslice - 1) / kslice;
+    a.kslice = kslice;
+    a.splitk = (int)(nsplit < 1 ? 1 : nsplit);
+    if (a.splitk > 1 && work == nullptr) return hipErrorInvalidValue;
+    a.W = work;
+    if (a.Ms <= 16) return launch_mt<T, 1>(a, s_kc, l_kc, trans_out, s);
+    if (a.Ms <= 32) return launch_mt<T, 2>(a, s_kc, l_kc, trans_out, s);
+    return launch_mt<T, 4>(a, s_kc, l_kc, trans_out, s);
+}
+
+template int skinny_pick_splitk<double>(const Gemm<double> &, int);
+template int skinny_pick_splitk<float>(const Gemm<float> &, int);
+template hipError_t launch_skinny<double>(const Gemm<double> &, int, double *, hipStream_t);
+template hipError_t launch_skinny<float>(const Gemm<float> &, int, float *, hipStream_t);
+
+}  // namespace blasdsp

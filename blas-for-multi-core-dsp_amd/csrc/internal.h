@@ -1,0 +1,47 @@
+// internal.h — shared declarations between the C-ABI layer and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace blasdsp {
+
+// One GEMM, already validated: C = alpha * op(A) op(B) + beta * C (row-major).
+template <typename T>
+struct Gemm {
+    bool ta, tb;          // op(A) = A^T, op(B) = B^T
+    int64_t M, N, K;
+    T alpha, beta;
+    const T *A;
+    int64_t lda;
+    const T *B;
+    int64_t ldb;
+    T *C;
+    int64_t ldc;
+};
+
+// MFMA tile kernel (gemm_tile.hip): any shape, any mode.
+template <typename T>
+hipError_t launch_tile(const Gemm<T> &g, hipStream_t s);
+
+// C = beta * C over M x N (beta == 0 writes zeros): K == 0 or alpha == 0.
+template <typename T>
+hipError_t launch_scale(const Gemm<T> &g, hipStream_t s);
+
+// Empty kernel, for launch-overhead measurement.
+hipError_t launch_null(hipStream_t s);
+
+// Skinny (shortA / thinB) kernels (gemm_skinny.hip).  `splitk` >= 1; when > 1
+// the partial sums go to `work` (splitk * M * N elements of T).
+template <typename T>
+hipError_t launch_skinny(const Gemm<T> &g, int splitk, T *work, hipStream_t s);
+// Number of K slices the skinny kernel would use for this shape on this GPU.
+template <typename T>
+int skinny_pick_splitk(const Gemm<T> &g, int num_cus);
+
+// Synthetic input generator (fill.hip).
+template <typename T>
+hipError_t launch_fill(T *X, int64_t rows, int64_t cols, int64_t ld, int trans, int dist,
+                       uint64_t seed, T pad, hipStream_t s);
+
+}  // namespace blasdsp

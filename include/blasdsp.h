@@ -1,0 +1,189 @@
+/*
+ * blasdsp.h — C ABI of the MI355X (gfx950) GEMM library.
+ *
+ * Every GEMM computes, row-major,
+ *
+ *     C = alpha * op(A) * op(B) + beta * C
+ *
+ * op(A) is M x K and op(B) is K x N.  For transA == 'N' A is stored M x K
+ * (lda >= K); for 'T' A is stored K x M (lda >= M).  Likewise B is K x N
+ * (ldb >= N) for 'N' and N x K (ldb >= K) for 'T'; C is M x N (ldc >= N).
+ * Element (i, j) of a matrix with leading dimension ld is X[i * ld + j], as in
+ * the reference (#define OFFSET(row, col, ld), dgemm_release_V1.1/device_code/
+ * dgemm_kernel.c:8).  When beta == 0, C is not read.
+ *
+ * A, B and C are device (HBM) pointers owned by the caller.  Every call is
+ * asynchronous on `stream` (a hipStream_t; NULL = the null stream) and
+ * returns a status code; errors detected on the host leave C untouched.
+ * Calls on distinct streams may run concurrently.
+ *
+ * Semantics relative to the reference.  The reference's dgemm and irr kernels
+ * compute C += op(A)op(B) (here: alpha = 1, beta = 1); sgemm V1.3.2 and
+ * irr_thinB_sgemm V1.2 compute C = alpha*op(A)op(B) + beta with a SCALAR beta
+ * and ignore the input C.  This library implements the BLAS form above for
+ * every entry point; the mapping is written out beside each declaration.
+ */
+#ifndef BLASDSP_H
+#define BLASDSP_H
+
+#include <stdint.h>
+
+#if defined(__GNUC__)
+#define BLASDSP_API __attribute__((visibility("default")))
+#else
+#define BLASDSP_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define BLASDSP_SUCCESS 0
+#define BLASDSP_ERR_INVALID (-22)     /* bad dimension, leading dim, mode or NULL */
+#define BLASDSP_ERR_NOMEM (-12)       /* workspace / budget allocation failed */
+#define BLASDSP_ERR_DEVICE (-5)       /* HIP runtime error (launch, copy, sync) */
+#define BLASDSP_ERR_UNSUPPORTED (-95) /* valid request the library does not serve */
+
+BLASDSP_API const char *blasdsp_version(void);
+BLASDSP_API const char *blasdsp_status_string(int status);
+/* Last HIP error code seen by the library on this thread (0 if none). */
+BLASDSP_API int blasdsp_last_hip_error(void);
+
+/* ---- general entry points ----------------------------------------------- */
+
+/* Dispatches by shape: the MFMA tile kernel for general shapes, the
+ * bandwidth-bound skinny kernel when M or N is small.  Replaces the per-mode
+ * __global__ functions of dgemm_release_V1.1/include/dgemm_kernel.h:11-38. */
+BLASDSP_API int blasdsp_dgemm(char transA, char transB, int64_t M, int64_t N, int64_t K,
+                  double alpha, const double *A, int64_t lda,
+                  const double *B, int64_t ldb, double beta,
+                  double *C, int64_t ldc, void *stream);
+
+/* FP32, FP32 accumulation on MFMA.  Replaces sgemm_NN_qxx ... sgemm_TT_qxx,
+ * sgemm_release_V1.3.2/include/sgemm_kernel.h:11-23. */
+BLASDSP_API int blasdsp_sgemm(char transA, char transB, int64_t M, int64_t N, int64_t K,
+                  float alpha, const float *A, int64_t lda,
+                  const float *B, int64_t ldb, float beta,
+                  float *C, int64_t ldc, void *stream);
+
+/* ---- per-mode entry points (reference names) ----------------------------
+ * Argument order follows the reference (lda, ldb, ldc, M, N, K, A, B, C) with
+ * the barrier ids dropped (no group barriers exist on the GPU) and alpha,
+ * beta, stream appended (alpha/beta last, as in sgemm_NN_qxx).
+ *   reference dgemm_NN(lda, ldb, ldc, M, N, K, b_id, A, B, C)   C += A B
+ *   ->        dgemm_NN(lda, ldb, ldc, M, N, K, A, B, C, 1.0, 1.0, stream)   */
+
+/* dgemm_release_V1.1/include/dgemm_kernel.h:11 */
+BLASDSP_API int dgemm_NN(int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N, int64_t K,
+             const double *A, const double *B, double *C, double alpha, double beta,
+             void *stream);
+/* dgemm_release_V1.1/include/dgemm_kernel.h:18 */
+BLASDSP_API int dgemm_NT(int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N, int64_t K,
+             const double *A, const double *B, double *C, double alpha, double beta,
+             void *stream);
+/* dgemm_release_V1.1/include/dgemm_kernel.h:25 (bid_main, bid_T dropped) */
+BLASDSP_API int dgemm_TN(int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N, int64_t K,
+             const double *A, const double *B, double *C, double alpha, double beta,
+             void *stream);
+/* dgemm_release_V1.1/include/dgemm_kernel.h:33 */
+BLASDSP_API int dgemm_TT(int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N, int64_t K,
+             const double *A, const double *B, double *C, double alpha, double beta,
+             void *stream);
+/* dgemm_release_V1.1/include/dgemm_kernel.h:40 — empty kernel launch, used to
+ * measure launch overhead as the reference harness does
+ * (dgemm_release_V1.1/host_code/dgemm_demo.c:140-146).  Arguments are checked
+ * like dgemm_NN's and otherwise ignored. */
+BLASDSP_API int dgemm_null(int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N, int64_t K,
+               const double *A, const double *B, double *C, double alpha, double beta,
+               void *stream);
+
+/* sgemm_release_V1.3.2/include/sgemm_kernel.h:11-26 (sgemm_*_qxx, packed
+ * lda = K, ldb = N, ldc = N there; explicit leading dimensions here).
+ * Reference C = alpha*op(A)op(B) + beta(scalar) with C ignored corresponds to
+ * beta = 0 here followed by adding the scalar. */
+BLASDSP_API int sgemm_NN(int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N, int64_t K,
+             const float *A, const float *B, float *C, float alpha, float beta,
+             void *stream);
+BLASDSP_API int sgemm_NT(int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N, int64_t K,
+             const float *A, const float *B, float *C, float alpha, float beta,
+             void *stream);
+BLASDSP_API int sgemm_TN(int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N, int64_t K,
+             const float *A, const float *B, float *C, float alpha, float beta,
+             void *stream);
+BLASDSP_API int sgemm_TT(int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N, int64_t K,
+             const float *A, const float *B, float *C, float alpha, float beta,
+             void *stream);
+/* sgemm_release_V1.3.2/include/sgemm_kernel.h:25 */
+BLASDSP_API int sgemm_null(int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N, int64_t K,
+               const float *A, const float *B, float *C, float alpha, float beta,
+               void *stream);
+
+/* ---- irregular shapes: shortA (small M) and thinB (small N) --------------
+ * Same arguments as dgemm_NN.  These always run the bandwidth-bound skinny
+ * kernel (op(B) or op(A) streamed once from HBM, the small operand held on
+ * chip, split-K with a deterministic reduction when the long side alone does
+ * not fill the GPU).  Any shape is accepted; they pay off for M <= 64
+ * (shortA) or N <= 64 (thinB).
+ *   irr_shortA_dgemm_release_V0.0/include/irr_shortA_dgemm_kernel.h:11-23
+ *     irr_shortA_dgemm_{NN,NT,TN,TT}_dbuf(M, N, K, b_id, A, B, C)   C += op(A)op(B)
+ *   irr_shortA_sgemm_release_V0.0/include/irr_shortA_sgemm_kernel.h:11-23
+ *   irr_thinB_dgemm_release_V0.0/include/irr_thinB_dgemm_kernel.h:11-23
+ *   irr_thinB_sgemm_release_V1.2/include/irr_thinB_sgemm_kernel.h:11-23
+ *     irr_thinB_sgemm_{NN,NT,TN,TT}(M, N, K, b_id, A, B, C, alpha, beta)  */
+#define BLASDSP_DECLARE_IRR(name, T)                                                    \
+    BLASDSP_API int name(int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N, int64_t K,    \
+             const T *A, const T *B, T *C, T alpha, T beta, void *stream);
+BLASDSP_DECLARE_IRR(irr_shortA_dgemm_NN, double)
+BLASDSP_DECLARE_IRR(irr_shortA_dgemm_NT, double)
+BLASDSP_DECLARE_IRR(irr_shortA_dgemm_TN, double)
+BLASDSP_DECLARE_IRR(irr_shortA_dgemm_TT, double)
+BLASDSP_DECLARE_IRR(irr_shortA_sgemm_NN, float)
+BLASDSP_DECLARE_IRR(irr_shortA_sgemm_NT, float)
+BLASDSP_DECLARE_IRR(irr_shortA_sgemm_TN, float)
+BLASDSP_DECLARE_IRR(irr_shortA_sgemm_TT, float)
+BLASDSP_DECLARE_IRR(irr_thinB_dgemm_NN, double)
+BLASDSP_DECLARE_IRR(irr_thinB_dgemm_NT, double)
+BLASDSP_DECLARE_IRR(irr_thinB_dgemm_TN, double)
+BLASDSP_DECLARE_IRR(irr_thinB_dgemm_TT, double)
+BLASDSP_DECLARE_IRR(irr_thinB_sgemm_NN, float)
+BLASDSP_DECLARE_IRR(irr_thinB_sgemm_NT, float)
+BLASDSP_DECLARE_IRR(irr_thinB_sgemm_TN, float)
+BLASDSP_DECLARE_IRR(irr_thinB_sgemm_TT, float)
+#undef BLASDSP_DECLARE_IRR
+
+/* ---- kernel selection (explicit) ----------------------------------------- */
+enum blasdsp_algo {
+    BLASDSP_ALGO_AUTO = 0,   /* shape-based dispatch (as blasdsp_dgemm)        */
+    BLASDSP_ALGO_TILE = 1,   /* MFMA tile kernel                               */
+    BLASDSP_ALGO_SKINNY = 2  /* bandwidth-bound skinny kernel (+ split-K)      */
+};
+/* splitk <= 0: chosen by the library.  Only the skinny kernel splits K. */
+BLASDSP_API int blasdsp_dgemm_ex(char transA, char transB, int64_t M, int64_t N, int64_t K,
+                     double alpha, const double *A, int64_t lda,
+                     const double *B, int64_t ldb, double beta,
+                     double *C, int64_t ldc, int algo, int splitk, void *stream);
+BLASDSP_API int blasdsp_sgemm_ex(char transA, char transB, int64_t M, int64_t N, int64_t K,
+                     float alpha, const float *A, int64_t lda,
+                     const float *B, int64_t ldb, float beta,
+                     float *C, int64_t ldc, int algo, int splitk, void *stream);
+
+/* ---- synthetic inputs ----------------------------------------------------
+ * Fill the logical rows x cols matrix stored at X (device memory, leading
+ * dimension ld, transposed storage when trans != 0) from the counter-based
+ * generator: element (i, j) takes draw number i * cols + j of the seeded
+ * stream, so the same logical matrix results for any ld or storage order.
+ * Padding columns are set to `pad`.  dist: 0 uniform[-1,1), 1 normal(0,1),
+ * 2 log-uniform magnitude 2^U[-8,8) with random sign, 3 (i*cols+j) % 11,
+ * 4 (i*cols+j) % 7, 5 ones.  Uniform and the integer patterns are bit-exact
+ * with the host generator; normal and log-uniform go through device
+ * log/cos/exp2 and may differ from the host generator in the last bit. */
+BLASDSP_API int blasdsp_fill_f64(double *X, int64_t rows, int64_t cols, int64_t ld, int trans,
+                     int dist, uint64_t seed, double pad, void *stream);
+BLASDSP_API int blasdsp_fill_f32(float *X, int64_t rows, int64_t cols, int64_t ld, int trans,
+                     int dist, uint64_t seed, float pad, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BLASDSP_H */

@@ -1,0 +1,413 @@
+/*
+ * oracle.c — CPU oracle for the GEMM hot path.  TEST INFRASTRUCTURE ONLY:
+ * loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg;
+ * never linked into or called by the product library.
+ *
+ * Build: oracle/Makefile (gcc -O3 -fopenmp -ffp-contract=off).  Contraction is
+ * off so that every expression rounds exactly as written, as the reference's
+ * host code does when built with -std=c99 (sgemm_release_V1.3.2/host_code/Makefile:5).
+ *
+ * Parity pins (tests/test_oracle.py):
+ *   - integer-pattern known answers (the reference demos' %11 / %7 matrices)
+ *     against an exact int64 matrix product;
+ *   - hand-derived rounding cases that fix the K_G / K_A chunk boundaries,
+ *     the tail chunk and where alpha is folded;
+ *   - exact rational (fractions.Fraction) brute force on tiny inputs for the
+ *     long-double oracle and the normwise error;
+ *   - closed forms (all-ones, identity, rank-1, K = 0) and mode invariance.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <omp.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define IDX(row, col, ld) ((int64_t)(row) * (int64_t)(ld) + (int64_t)(col))
+
+static int g_threads = 0; /* 0: OpenMP default */
+
+void oracle_set_threads(int n) { g_threads = n > 0 ? n : 0; }
+int oracle_get_threads(void) { return g_threads > 0 ? g_threads : omp_get_max_threads(); }
+
+/* ------------------------------------------------------------------------- */
+/* 1. generators                                                             */
+/* ------------------------------------------------------------------------- */
+
+/* splitmix64 finaliser (Steele, Lea, Flood 2014). */
+uint64_t oracle_mix64(uint64_t x)
+{
+    x ^= x >> 30;
+    x *= 0xBF58476D1CE4E5B9ULL;
+    x ^= x >> 27;
+    x *= 0x94D049BB133111EBULL;
+    x ^= x >> 31;
+    return x;
+}
+
+/* Output number c of a splitmix64 stream whose state starts at a key derived
+ * from the seed: counter-based, so any element can be produced on its own. */
+static uint64_t stream_bits(uint64_t seed, uint64_t c)
+{
+    uint64_t key = oracle_mix64(seed * 0x9E3779B97F4A7C15ULL + 0x632BE59BD9B4E019ULL);
+    return oracle_mix64(key + (c + 1) * 0x9E3779B97F4A7C15ULL);
+}
+
+/* [0, 1) with 53 random bits: m * 2^-53. */
+static double unit_open_hi(uint64_t bits) { return (double)(bits >> 11) * 0x1.0p-53; }
+
+double oracle_value(int dist, uint64_t seed, uint64_t idx)
+{
+    switch (dist) {
+    case ORACLE_UNIFORM:
+        /* 2u - 1 is exact in double. */
+        return 2.0 * unit_open_hi(stream_bits(seed, idx)) - 1.0;
+    case ORACLE_NORMAL: {
+        double u1 = 1.0 - unit_open_hi(stream_bits(seed, 2 * idx));     /* (0, 1] */
+        double u2 = unit_open_hi(stream_bits(seed, 2 * idx + 1));
+        return sqrt(-2.0 * log(u1)) * cos(6.283185307179586476925 * u2);
+    }
+    case ORACLE_LOGUNIF: {
+        uint64_t b0 = stream_bits(seed, 2 * idx);
+        double u = unit_open_hi(stream_bits(seed, 2 * idx + 1));
+        double mag = exp2(16.0 * u - 8.0);
+        return (b0 >> 63) ? -mag : mag;
+    }
+    case ORACLE_PATTERN11:
+        return (double)(idx % 11);
+    case ORACLE_PATTERN7:
+        return (double)(idx % 7);
+    case ORACLE_ONES:
+        return 1.0;
+    default:
+        return 0.0;
+    }
+}
+
+void oracle_fill_f64(double *X, int64_t rows, int64_t cols, int64_t ld, int trans,
+                     int dist, uint64_t seed, double pad)
+{
+    int64_t srows = trans ? cols : rows; /* stored rows */
+    int64_t scols = trans ? rows : cols; /* stored logical columns */
+#pragma omp parallel for num_threads(oracle_get_threads()) schedule(static)
+    for (int64_t r = 0; r < srows; r++) {
+        for (int64_t c = 0; c < ld; c++) {
+            double v = pad;
+            if (c < scols) {
+                int64_t i = trans ? c : r, j = trans ? r : c;
+                v = oracle_value(dist, seed, (uint64_t)(i * cols + j));
+            }
+            X[IDX(r, c, ld)] = v;
+        }
+    }
+}
+
+void oracle_fill_f32(float *X, int64_t rows, int64_t cols, int64_t ld, int trans,
+                     int dist, uint64_t seed, float pad)
+{
+    int64_t srows = trans ? cols : rows;
+    int64_t scols = trans ? rows : cols;
+#pragma omp parallel for num_threads(oracle_get_threads()) schedule(static)
+    for (int64_t r = 0; r < srows; r++) {
+        for (int64_t c = 0; c < ld; c++) {
+            float v = pad;
+            if (c < scols) {
+                int64_t i = trans ? c : r, j = trans ? r : c;
+                v = (float)oracle_value(dist, seed, (uint64_t)(i * cols + j));
+            }
+            X[IDX(r, c, ld)] = v;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* 2. restated reference host loops                                          */
+/* ------------------------------------------------------------------------- */
+
+/* Follows dgemm_release_V1.1/host_code/dgemm_demo.c:233-252 (same loop in
+ * irr_shortA_dgemm_release_V0.0/host_code/irr_shortA_dgemm_demo.c:205-224 and
+ * irr_thinB_dgemm_release_V0.0/host_code/irr_thinB_dgemm_demo.c).  Per
+ * element: every full K_G chunk is an fma chain from 0.0 added into C, then
+ * the tail chunk k in [K - K%K_G, K) likewise, added even when empty. */
+void oracle_dgemm_v11_host(int64_t M, int64_t N, int64_t K,
+                           const double *A, int64_t lda, const double *B, int64_t ldb,
+                           double *C, int64_t ldc, int64_t K_G)
+{
+#pragma omp parallel for num_threads(oracle_get_threads()) collapse(2) schedule(static)
+    for (int64_t i = 0; i < M; i++) {
+        for (int64_t j = 0; j < N; j++) {
+            for (int64_t k = 0; k < K - K % K_G; k += K_G) {
+                double tmp = 0.0;
+                for (int64_t kk = 0; kk < K_G; kk++)
+                    tmp = fma(A[IDX(i, k + kk, lda)], B[IDX(k + kk, j, ldb)], tmp);
+                C[IDX(i, j, ldc)] += tmp;
+            }
+            double tmp = 0.0;
+            for (int64_t k = K - K % K_G; k < K; k++)
+                tmp = fma(A[IDX(i, k, lda)], B[IDX(k, j, ldb)], tmp);
+            C[IDX(i, j, ldc)] += tmp;
+        }
+    }
+}
+
+/* Follows irr_shortA_sgemm_release_V0.0/host_code/irr_shortA_sgemm_demo.c:205-224:
+ * the loop above with dataType float — tmp is a float, fma() runs in double
+ * and its result is rounded to float on assignment. */
+void oracle_sgemm_plain_host(int64_t M, int64_t N, int64_t K,
+                             const float *A, int64_t lda, const float *B, int64_t ldb,
+                             float *C, int64_t ldc, int64_t K_G)
+{
+#pragma omp parallel for num_threads(oracle_get_threads()) collapse(2) schedule(static)
+    for (int64_t i = 0; i < M; i++) {
+        for (int64_t j = 0; j < N; j++) {
+            for (int64_t k = 0; k < K - K % K_G; k += K_G) {
+                float tmp = 0.0f;
+                for (int64_t kk = 0; kk < K_G; kk++)
+                    tmp = (float)fma((double)A[IDX(i, k + kk, lda)],
+                                     (double)B[IDX(k + kk, j, ldb)], (double)tmp);
+                C[IDX(i, j, ldc)] += tmp;
+            }
+            float tmp = 0.0f;
+            for (int64_t k = K - K % K_G; k < K; k++)
+                tmp = (float)fma((double)A[IDX(i, k, lda)], (double)B[IDX(k, j, ldb)],
+                                 (double)tmp);
+            C[IDX(i, j, ldc)] += tmp;
+        }
+    }
+}
+
+/* Follows sgemm_release_V1.3.2/host_code/sgemm_demo.c:214-242.  alpha * A is
+ * a float product (rounded to float) before it enters the double fma; after
+ * the chunks, the scalar beta is added (the reference's C_plus_beta semantic,
+ * sgemm_release_V1.3.2/device_code/sgemm_kernel.c:25-160). */
+void oracle_sgemm_v132_host(int64_t M, int64_t N, int64_t K, float alpha, float beta,
+                            const float *A, int64_t lda, const float *B, int64_t ldb,
+                            float *C, int64_t ldc, int64_t K_G)
+{
+#pragma omp parallel for num_threads(oracle_get_threads()) collapse(2) schedule(static)
+    for (int64_t i = 0; i < M; i++) {
+        for (int64_t j = 0; j < N; j++) {
+            for (int64_t k = 0; k < K - K % K_G; k += K_G) {
+                float tmp = 0.0f;
+                for (int64_t kk = 0; kk < K_G; kk++) {
+                    float a = alpha * A[IDX(i, k + kk, lda)];
+                    tmp = (float)fma((double)a, (double)B[IDX(k + kk, j, ldb)], (double)tmp);
+                }
+                C[IDX(i, j, ldc)] += tmp;
+            }
+            float tmp = 0.0f;
+            for (int64_t k = K - K % K_G; k < K; k++) {
+                float a = alpha * A[IDX(i, k, lda)];
+                tmp = (float)fma((double)a, (double)B[IDX(k, j, ldb)], (double)tmp);
+            }
+            C[IDX(i, j, ldc)] += tmp;
+            C[IDX(i, j, ldc)] += beta;
+        }
+    }
+}
+
+static int64_t min64(int64_t a, int64_t b) { return a < b ? a : b; }
+
+/* Follows irr_thinB_sgemm_release_V1.2/host_code/irr_thinB_sgemm_demo.c:217-240:
+ * a three-level tree (K_G outer chunks of K_A inner chunks, each inner chunk an
+ * fma chain from 0), alpha folded into the B operand, and C += tmp_o + beta. */
+void oracle_thinB_sgemm_v12_host(int64_t M, int64_t N, int64_t K, float alpha, float beta,
+                                 const float *A, int64_t lda, const float *B, int64_t ldb,
+                                 float *C, int64_t ldc, int64_t K_G, int64_t K_A)
+{
+#pragma omp parallel for num_threads(oracle_get_threads()) collapse(2) schedule(static)
+    for (int64_t i = 0; i < M; i++) {
+        for (int64_t j = 0; j < N; j++) {
+            float tmp_o = 0.0f;
+            for (int64_t k_o = 0; k_o < K; k_o += K_G) {
+                int64_t kg = min64(K_G, K - k_o);
+                float tmp_i = 0.0f;
+                for (int64_t k_i = 0; k_i < kg; k_i += K_A) {
+                    int64_t ka = min64(K_A, kg - k_i);
+                    float tmp_s = 0.0f;
+                    for (int64_t k_s = 0; k_s < ka; k_s++) {
+                        int64_t k = k_o + k_i + k_s;
+                        float b = alpha * B[IDX(k, j, ldb)];
+                        tmp_s = (float)fma((double)A[IDX(i, k, lda)], (double)b, (double)tmp_s);
+                    }
+                    tmp_i += tmp_s;
+                }
+                tmp_o += tmp_i;
+            }
+            C[IDX(i, j, ldc)] += tmp_o + beta;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* 3. BLAS-epilogue chunked loop = CPU baseline                              */
+/* ------------------------------------------------------------------------- */
+
+/* The accumulation tree of dgemm_demo.c:233-252 (K_G chunks, fma chain from
+ * 0, chunks summed in order), finished with the BLAS epilogue
+ * C = alpha * acc + beta * C that the product entry points implement. */
+void oracle_dgemm_blas_host(int64_t M, int64_t N, int64_t K, double alpha,
+                            const double *A, int64_t lda, const double *B, int64_t ldb,
+                            double beta, double *C, int64_t ldc, int64_t K_G)
+{
+#pragma omp parallel for num_threads(oracle_get_threads()) collapse(2) schedule(static)
+    for (int64_t i = 0; i < M; i++) {
+        for (int64_t j = 0; j < N; j++) {
+            double acc = 0.0;
+            for (int64_t k = 0; k < K - K % K_G; k += K_G) {
+                double tmp = 0.0;
+                for (int64_t kk = 0; kk < K_G; kk++)
+                    tmp = fma(A[IDX(i, k + kk, lda)], B[IDX(k + kk, j, ldb)], tmp);
+                acc += tmp;
+            }
+            double tmp = 0.0;
+            for (int64_t k = K - K % K_G; k < K; k++)
+                tmp = fma(A[IDX(i, k, lda)], B[IDX(k, j, ldb)], tmp);
+            acc += tmp;
+            double out = alpha * acc;
+            if (beta != 0.0) out = out + beta * C[IDX(i, j, ldc)];
+            C[IDX(i, j, ldc)] = out;
+        }
+    }
+}
+
+/* FP32 form (float tmp, double fma, as sgemm_demo.c:214-242 without alpha
+ * folding), BLAS epilogue in float. */
+void oracle_sgemm_blas_host(int64_t M, int64_t N, int64_t K, float alpha,
+                            const float *A, int64_t lda, const float *B, int64_t ldb,
+                            float beta, float *C, int64_t ldc, int64_t K_G)
+{
+#pragma omp parallel for num_threads(oracle_get_threads()) collapse(2) schedule(static)
+    for (int64_t i = 0; i < M; i++) {
+        for (int64_t j = 0; j < N; j++) {
+            float acc = 0.0f;
+            for (int64_t k = 0; k < K - K % K_G; k += K_G) {
+                float tmp = 0.0f;
+                for (int64_t kk = 0; kk < K_G; kk++)
+                    tmp = (float)fma((double)A[IDX(i, k + kk, lda)],
+                                     (double)B[IDX(k + kk, j, ldb)], (double)tmp);
+                acc += tmp;
+            }
+            float tmp = 0.0f;
+            for (int64_t k = K - K % K_G; k < K; k++)
+                tmp = (float)fma((double)A[IDX(i, k, lda)], (double)B[IDX(k, j, ldb)],
+                                 (double)tmp);
+            acc += tmp;
+            float out = alpha * acc;
+            if (beta != 0.0f) out = out + beta * C[IDX(i, j, ldc)];
+            C[IDX(i, j, ldc)] = out;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* 4. extended-precision oracle                                              */
+/* ------------------------------------------------------------------------- */
+
+/* op(B) packed column-major (column j contiguous over k) so the inner loop
+ * streams; accumulation in x87 long double (64-bit significand). */
+void oracle_gemm_xp_f64(char transA, char transB, int64_t M, int64_t N, int64_t K,
+                        double alpha, const double *A, int64_t lda,
+                        const double *B, int64_t ldb, double beta,
+                        double *C, int64_t ldc)
+{
+    int ta = (transA == 'T' || transA == 't');
+    int tb = (transB == 'T' || transB == 't');
+    double *Bp = (double *)malloc(sizeof(double) * (size_t)(K > 0 ? K : 1) * (size_t)(N > 0 ? N : 1));
+    if (!Bp) return;
+#pragma omp parallel for num_threads(oracle_get_threads()) schedule(static)
+    for (int64_t j = 0; j < N; j++)
+        for (int64_t k = 0; k < K; k++)
+            Bp[IDX(j, k, K)] = tb ? B[IDX(j, k, ldb)] : B[IDX(k, j, ldb)];
+#pragma omp parallel num_threads(oracle_get_threads())
+    {
+        double *arow = (double *)malloc(sizeof(double) * (size_t)(K > 0 ? K : 1));
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < M; i++) {
+            for (int64_t k = 0; k < K; k++)
+                arow[k] = ta ? A[IDX(k, i, lda)] : A[IDX(i, k, lda)];
+            for (int64_t j = 0; j < N; j++) {
+                const double *bc = Bp + IDX(j, 0, K);
+                long double acc = 0.0L;
+                for (int64_t k = 0; k < K; k++) acc += (long double)arow[k] * (long double)bc[k];
+                long double out = (long double)alpha * acc;
+                if (beta != 0.0) out += (long double)beta * (long double)C[IDX(i, j, ldc)];
+                C[IDX(i, j, ldc)] = (double)out;
+            }
+        }
+        free(arow);
+    }
+    free(Bp);
+}
+
+void oracle_gemm_xp_f32(char transA, char transB, int64_t M, int64_t N, int64_t K,
+                        double alpha, const float *A, int64_t lda,
+                        const float *B, int64_t ldb, double beta,
+                        const float *C32, int64_t ldc32, double *C64, int64_t ldc64)
+{
+    int ta = (transA == 'T' || transA == 't');
+    int tb = (transB == 'T' || transB == 't');
+    double *Bp = (double *)malloc(sizeof(double) * (size_t)(K > 0 ? K : 1) * (size_t)(N > 0 ? N : 1));
+    if (!Bp) return;
+#pragma omp parallel for num_threads(oracle_get_threads()) schedule(static)
+    for (int64_t j = 0; j < N; j++)
+        for (int64_t k = 0; k < K; k++)
+            Bp[IDX(j, k, K)] = tb ? (double)B[IDX(j, k, ldb)] : (double)B[IDX(k, j, ldb)];
+#pragma omp parallel num_threads(oracle_get_threads())
+    {
+        double *arow = (double *)malloc(sizeof(double) * (size_t)(K > 0 ? K : 1));
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < M; i++) {
+            for (int64_t k = 0; k < K; k++)
+                arow[k] = ta ? (double)A[IDX(k, i, lda)] : (double)A[IDX(i, k, lda)];
+            for (int64_t j = 0; j < N; j++) {
+                const double *bc = Bp + IDX(j, 0, K);
+                long double acc = 0.0L;
+                for (int64_t k = 0; k < K; k++) acc += (long double)arow[k] * (long double)bc[k];
+                long double out = (long double)alpha * acc;
+                if (beta != 0.0) out += (long double)beta * (long double)C32[IDX(i, j, ldc32)];
+                C64[IDX(i, j, ldc64)] = (double)out;
+            }
+        }
+        free(arow);
+    }
+    free(Bp);
+}
+
+double oracle_normwise_f64(int64_t M, int64_t N, const double *X, int64_t ldx,
+                           const double *R, int64_t ldr, double *maxabs)
+{
+    long double num = 0.0L, den = 0.0L;
+    double mx = 0.0;
+    for (int64_t i = 0; i < M; i++)
+        for (int64_t j = 0; j < N; j++) {
+            long double r = R[IDX(i, j, ldr)];
+            long double d = (long double)X[IDX(i, j, ldx)] - r;
+            num += d * d;
+            den += r * r;
+            double ad = fabs((double)d);
+            if (ad > mx || ad != ad) mx = ad;
+        }
+    if (maxabs) *maxabs = mx;
+    if (den == 0.0L) return num == 0.0L ? 0.0 : INFINITY;
+    return (double)sqrtl(num / den);
+}
+
+double oracle_normwise_f32(int64_t M, int64_t N, const float *X, int64_t ldx,
+                           const double *R, int64_t ldr, double *maxabs)
+{
+    long double num = 0.0L, den = 0.0L;
+    double mx = 0.0;
+    for (int64_t i = 0; i < M; i++)
+        for (int64_t j = 0; j < N; j++) {
+            long double r = R[IDX(i, j, ldr)];
+            long double d = (long double)X[IDX(i, j, ldx)] - r;
+            num += d * d;
+            den += r * r;
+            double ad = fabs((double)d);
+            if (ad > mx || ad != ad) mx = ad;
+        }
+    if (maxabs) *maxabs = mx;
+    if (den == 0.0L) return num == 0.0L ? 0.0 : INFINITY;
+    return (double)sqrtl(num / den);
+}

@@ -1,0 +1,76 @@
+"""The C-ABI library loads, exports every function include/blasdsp.h declares,
+and rejects bad arguments on the host (no GPU needed: argument checks run
+before any HIP call)."""
+import ctypes
+
+import pytest
+
+
+def test_library_exports_every_declared_function(blas):
+    L = blas.lib()
+    names = blas.header_functions()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    for expected in ("dgemm_NN", "dgemm_TT", "sgemm_NT", "irr_shortA_dgemm_TN", "irr_thinB_sgemm_TT",
+                     "dgemm_null", "blasdsp_dgemm", "blasdsp_sgemm_ex", "blasdsp_fill_f32"):
+        assert expected in names
+
+
+def test_version_and_status_strings(blas):
+    assert "gfx950" in blas.version()
+    L = blas.lib()
+    assert L.blasdsp_status_string(0) == b"success"
+    assert L.blasdsp_status_string(-22) == b"invalid argument"
+
+
+FAKE = ctypes.c_void_p(0x1000)  # never dereferenced: every case fails the host checks
+
+
+@pytest.mark.parametrize("args", [
+    # (transA, transB, M, N, K, lda, ldb, ldc)
+    ("X", "N", 4, 4, 4, 4, 4, 4),   # bad mode
+    ("N", "Q", 4, 4, 4, 4, 4, 4),
+    ("N", "N", -1, 4, 4, 4, 4, 4),  # negative dims
+    ("N", "N", 4, -1, 4, 4, 4, 4),
+    ("N", "N", 4, 4, -1, 4, 4, 4),
+    ("N", "N", 4, 4, 8, 7, 4, 4),   # lda < K
+    ("T", "N", 9, 4, 4, 8, 4, 4),   # lda < M (A stored K x M)
+    ("N", "N", 4, 9, 4, 4, 8, 9),   # ldb < N
+    ("N", "T", 4, 4, 9, 9, 8, 4),   # ldb < K (B stored N x K)
+    ("N", "N", 4, 9, 4, 4, 9, 8),   # ldc < N
+])
+def test_invalid_arguments_rejected(blas, args):
+    ta, tb, M, N, K, lda, ldb, ldc = args
+    L = blas.lib()
+    for fn in (L.blasdsp_dgemm, L.blasdsp_sgemm):
+        st = fn(ta.encode(), tb.encode(), M, N, K, 1.0, FAKE, lda, FAKE, ldb, 0.0, FAKE, ldc, None)
+        assert st == blas.ERR_INVALID
+
+
+def test_null_pointers_rejected(blas):
+    L = blas.lib()
+    st = L.blasdsp_dgemm(b"N", b"N", 4, 4, 4, 1.0, None, 4, FAKE, 4, 0.0, FAKE, 4, None)
+    assert st == blas.ERR_INVALID
+    st = L.blasdsp_dgemm(b"N", b"N", 4, 4, 4, 1.0, FAKE, 4, FAKE, 4, 0.0, None, 4, None)
+    assert st == blas.ERR_INVALID
+    st = L.dgemm_NN(4, 4, 4, 4, 4, 4, FAKE, None, FAKE, 1.0, 1.0, None)
+    assert st == blas.ERR_INVALID
+    st = L.irr_thinB_sgemm_NN(4, 4, 4, 4, 4, 4, FAKE, FAKE, None, 1.0, 1.0, None)
+    assert st == blas.ERR_INVALID
+
+
+def test_empty_problems_are_noops(blas):
+    # M == 0 or N == 0: nothing to do, nothing launched, pointers unused
+    L = blas.lib()
+    assert L.blasdsp_dgemm(b"N", b"N", 0, 5, 5, 1.0, None, 5, None, 5, 0.0, None, 5, None) == 0
+    assert L.blasdsp_sgemm(b"T", b"T", 5, 0, 5, 1.0, None, 5, None, 5, 0.0, None, 1, None) == 0
+    assert L.sgemm_TT(5, 5, 1, 0, 0, 5, None, None, None, 1.0, 0.0, None) == 0
+
+
+def test_bad_algo_and_fill_arguments(blas):
+    L = blas.lib()
+    st = L.blasdsp_dgemm_ex(b"N", b"N", 4, 4, 4, 1.0, FAKE, 4, FAKE, 4, 0.0, FAKE, 4, 7, 0, None)
+    assert st == blas.ERR_INVALID or st == blas.ERR_DEVICE  # unknown algo
+    assert L.blasdsp_fill_f64(FAKE, 4, 4, 3, 0, 0, 1, 0.0, None) == blas.ERR_INVALID
+    assert L.blasdsp_fill_f32(FAKE, 4, 4, 4, 0, 9, 1, 0.0, None) == blas.ERR_INVALID

@@ -1,0 +1,249 @@
+"""GPU parity: the HIP kernels, called through the C ABI, against the CPU oracle.
+
+Tolerances (normwise ||C - C_ref||_F / ||C_ref||_F against the
+extended-precision oracle):
+  FP64: 1e-13.
+  FP32: (sqrt(K) + 4) * 2^-23 — sqrt(K) * eps32 for the K-term dot products plus
+        a few ulps for the alpha/beta epilogue.
+Integer-pattern inputs (the reference demos' %11 / %7 matrices) are exact in
+both precisions and are compared bit for bit.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import to_dev
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-13
+
+
+def tol32(K):
+    return (math.sqrt(max(K, 1)) + 4) * 2.0 ** -23
+
+
+def dtype_np(prec):
+    return np.float64 if prec == "d" else np.float32
+
+
+def stored(orc, rows, cols, trans, dist, seed, prec, pad=3):
+    """op(X) is rows x cols; returns the stored (maybe transposed) padded view."""
+    scols = rows if trans else cols
+    return orc.matrix(rows, cols, dist, seed, dtype=dtype_np(prec), ld=scols + pad, trans=trans)
+
+
+def run_case(orc, blas, dev, prec, ta, tb, M, N, K, alpha, beta, algo, dist=0, seeds=(1, 2, 3), splitk=0):
+    import torch
+
+    A = stored(orc, M, K, ta == "T", dist, seeds[0], prec)
+    B = stored(orc, K, N, tb == "T", dist, seeds[1], prec)
+    C0 = orc.matrix(M, N, dist if dist in (3, 4) else 0, seeds[2], dtype=dtype_np(prec), ld=N + 5)
+    ref = orc.gemm_xp(ta, tb, A, B, C0, alpha, beta)
+    dA, dB, dC = to_dev(A, dev), to_dev(B, dev), to_dev(C0, dev)
+    blas.gemm(ta, tb, dA, dB, dC, alpha, beta, algo=algo, splitk=splitk)
+    torch.cuda.synchronize()
+    got = np.ascontiguousarray(dC.cpu().numpy())
+    return got, ref
+
+
+SMALL_SHAPES = [(1, 1, 1), (7, 5, 3), (16, 16, 4), (17, 33, 65), (128, 128, 16), (129, 130, 131),
+                (200, 136, 300), (64, 300, 97)]
+
+
+@pytest.mark.parametrize("prec", ["d", "s"])
+@pytest.mark.parametrize("mode", ["NN", "NT", "TN", "TT"])
+@pytest.mark.parametrize("shape", SMALL_SHAPES)
+def test_tile_kernel_modes(orc, blas, dev, prec, mode, shape):
+    M, N, K = shape
+    got, ref = run_case(orc, blas, dev, prec, mode[0], mode[1], M, N, K, 1.5, -0.5, blas.ALGO_TILE)
+    err, _ = orc.normwise(got, ref)
+    assert err <= (TOL64 if prec == "d" else tol32(K)), err
+
+
+@pytest.mark.parametrize("prec", ["d", "s"])
+@pytest.mark.parametrize("mode", ["NN", "NT", "TN", "TT"])
+@pytest.mark.parametrize("shape", [(8, 300, 1000), (16, 64, 16), (32, 1000, 777), (40, 129, 64),
+                                   (1, 70, 33), (300, 8, 1000), (1000, 32, 777), (129, 40, 64), (70, 1, 33)])
+def test_skinny_kernel_modes(orc, blas, dev, prec, mode, shape):
+    M, N, K = shape
+    got, ref = run_case(orc, blas, dev, prec, mode[0], mode[1], M, N, K, 1.5, -0.5, blas.ALGO_SKINNY)
+    err, _ = orc.normwise(got, ref)
+    assert err <= (TOL64 if prec == "d" else tol32(K)), err
+
+
+@pytest.mark.parametrize("splitk", [1, 2, 3, 7])
+def test_skinny_splitk(orc, blas, dev, splitk):
+    for prec in "ds":
+        got, ref = run_case(orc, blas, dev, prec, "N", "N", 24, 520, 1111, 1.0, 1.0, blas.ALGO_SKINNY,
+                            splitk=splitk)
+        err, _ = orc.normwise(got, ref)
+        assert err <= (TOL64 if prec == "d" else tol32(1111)), (prec, err)
+
+
+@pytest.mark.parametrize("algo", ["tile", "skinny"])
+@pytest.mark.parametrize("mode", ["NN", "NT", "TN", "TT"])
+def test_integer_patterns_exact(orc, blas, dev, algo, mode):
+    """The reference demos' integer matrices: every partial sum is an integer
+    below 2^24, so both precisions must reproduce the exact product."""
+    a = blas.ALGO_TILE if algo == "tile" else blas.ALGO_SKINNY
+    M, N, K = (32, 200, 1300) if algo == "skinny" else (150, 130, 1300)
+    for prec in "ds":
+        A = stored(orc, M, K, mode[0] == "T", orc.PATTERN11, 0, prec)
+        B = stored(orc, K, N, mode[1] == "T", orc.PATTERN7, 0, prec)
+        P = (orc.matrix(M, K, orc.PATTERN11, 0).astype(np.int64)
+             @ orc.matrix(K, N, orc.PATTERN7, 0).astype(np.int64))
+        C0 = orc.matrix(M, N, orc.PATTERN7, 5, dtype=dtype_np(prec))
+        import torch
+        dA, dB, dC = to_dev(A, dev), to_dev(B, dev), to_dev(C0, dev)
+        blas.gemm(mode[0], mode[1], dA, dB, dC, 5.0, 3.0, algo=a)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(dC.cpu().numpy(), 5 * P + 3 * C0.astype(np.int64))
+
+
+def test_reference_entry_points(orc, blas, dev):
+    """Per-mode entry points with reference names and argument order; the
+    reference's C += op(A)op(B) is alpha = beta = 1."""
+    import torch
+
+    L = blas.lib()
+    M, N, K = 96, 80, 700
+    P = (orc.matrix(M, K, orc.PATTERN11, 0).astype(np.int64)
+         @ orc.matrix(K, N, orc.PATTERN7, 0).astype(np.int64))
+    s = torch.cuda.current_stream().cuda_stream
+    for prec, T in (("d", "dgemm"), ("s", "sgemm")):
+        for mode in ("NN", "NT", "TN", "TT"):
+            for fam in ("", "irr_shortA_", "irr_thinB_"):
+                A = to_dev(stored(orc, M, K, mode[0] == "T", orc.PATTERN11, 0, prec), dev)
+                B = to_dev(stored(orc, K, N, mode[1] == "T", orc.PATTERN7, 0, prec), dev)
+                C = to_dev(orc.matrix(M, N, orc.PATTERN7, 5, dtype=dtype_np(prec), ld=N + 1), dev)
+                C_in = C.cpu().numpy().astype(np.int64)
+                fn = getattr(L, f"{fam}{T}_{mode}")
+                st = fn(A.stride(0), B.stride(0), C.stride(0), M, N, K, A.data_ptr(), B.data_ptr(),
+                        C.data_ptr(), 1.0, 1.0, s)
+                assert st == 0
+                torch.cuda.synchronize()
+                np.testing.assert_array_equal(C.cpu().numpy(), P + C_in, err_msg=f"{fam}{T}_{mode}")
+    assert L.dgemm_null(K, N, N, M, N, K, A.data_ptr(), B.data_ptr(), C.data_ptr(), 1.0, 1.0, s) == 0
+
+
+def test_beta_zero_ignores_c_and_degenerate_k(orc, blas, dev):
+    import torch
+
+    for prec, dt in (("d", torch.float64), ("s", torch.float32)):
+        A = to_dev(stored(orc, 40, 50, False, 0, 1, prec), dev)
+        B = to_dev(stored(orc, 50, 60, False, 0, 2, prec), dev)
+        for algo in (blas.ALGO_TILE, blas.ALGO_SKINNY):
+            C = torch.full((40, 60), float("nan"), dtype=dt, device=dev)
+            blas.gemm("N", "N", A, B, C, 1.0, 0.0, algo=algo)
+            torch.cuda.synchronize()
+            assert torch.isfinite(C).all()
+        # K = 0 and alpha = 0: C = beta C
+        C0 = torch.arange(40 * 60, dtype=dt, device=dev).reshape(40, 60)
+        C = C0.clone()
+        blas.gemm("N", "N", A[:, :0], B[:0, :], C, 1.0, 2.0)
+        torch.cuda.synchronize()
+        assert torch.equal(C, 2 * C0)
+        C = C0.clone()
+        blas.gemm("N", "N", A, B, C, 0.0, -1.0)
+        torch.cuda.synchronize()
+        assert torch.equal(C, -C0)
+        C = torch.full((40, 60), float("nan"), dtype=dt, device=dev)
+        blas.gemm("N", "N", A, B, C, 0.0, 0.0)
+        torch.cuda.synchronize()
+        assert torch.equal(C, torch.zeros_like(C))
+
+
+def test_fill_matches_host_generator(orc, blas, dev):
+    import torch
+
+    for dist in (orc.UNIFORM, orc.PATTERN11, orc.PATTERN7, orc.ONES):
+        for trans in (False, True):
+            ref = orc.matrix(33, 47, dist, 9, ld=(33 if trans else 47) + 2, trans=trans, pad=0.0)
+            X = torch.empty(ref.base.shape, dtype=torch.float64, device=dev)
+            blas.fill(X, 33, 47, dist, 9, trans=trans, pad=0.0)
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(X.cpu().numpy(), ref.base)
+            X32 = torch.empty(ref.base.shape, dtype=torch.float32, device=dev)
+            blas.fill(X32, 33, 47, dist, 9, trans=trans, pad=0.0)
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(X32.cpu().numpy(), ref.base.astype(np.float32))
+    for dist in (orc.NORMAL, orc.LOGUNIF):
+        ref = orc.matrix(64, 64, dist, 4)
+        X = torch.empty((64, 64), dtype=torch.float64, device=dev)
+        blas.fill(X, 64, 64, dist, 4)
+        torch.cuda.synchronize()
+        np.testing.assert_allclose(X.cpu().numpy(), ref, rtol=1e-13, atol=1e-15)
+
+
+def test_dgemm_4096_all_modes(orc, blas, dev):
+    """DGEMM 4096^3 with lda = ldb = ldc = 4104, alpha = 1.5, beta = -0.5,
+    U[-1,1] from seed 1: 32 rows of every mode against the oracle, and the four
+    modes against each other over the whole matrix."""
+    import torch
+
+    n, ld = 4096, 4104
+    alpha, beta = 1.5, -0.5
+    outs = {}
+    for mode in ("NN", "NT", "TN", "TT"):
+        A = torch.empty((n, ld), dtype=torch.float64, device=dev)
+        B = torch.empty((n, ld), dtype=torch.float64, device=dev)
+        C = torch.empty((n, ld), dtype=torch.float64, device=dev)
+        blas.fill(A, n, n, 0, 11, trans=mode[0] == "T")
+        blas.fill(B, n, n, 0, 12, trans=mode[1] == "T")
+        blas.fill(C, n, n, 0, 13)
+        blas.gemm(mode[0], mode[1], A[:, :n], B[:, :n], C[:, :n], alpha, beta)
+        torch.cuda.synchronize()
+        outs[mode] = C[:, :n].cpu().numpy()
+        del A, B, C
+    r0, nr = 1000, 32
+    A_rows = np.ascontiguousarray(orc.matrix(n, n, 0, 11)[r0:r0 + nr])
+    B_full = orc.matrix(n, n, 0, 12)
+    C_rows = np.ascontiguousarray(orc.matrix(n, n, 0, 13)[r0:r0 + nr])
+    ref = orc.gemm_xp("N", "N", A_rows, B_full, C_rows, alpha, beta)
+    for mode, C in outs.items():
+        err, _ = orc.normwise(np.ascontiguousarray(C[r0:r0 + nr]), ref)
+        assert err <= TOL64, (mode, err)
+        merr, _ = orc.normwise(C, outs["NN"])
+        assert merr <= 1e-15, (mode, merr)
+
+
+@pytest.mark.parametrize("prec", ["s", "d"])
+@pytest.mark.parametrize("kind", ["shortA", "thinB"])
+def test_skinny_full_size(orc, blas, dev, prec, kind):
+    """shortA M = 32, K = 8192, N = 65536 (thinB: M and N swapped), N(0,1) from
+    seed 2, alpha = beta = 1: 96 columns (rows) against the oracle; the
+    automatic split-K result against a single-slice run."""
+    import torch
+
+    dt = torch.float64 if prec == "d" else torch.float32
+    small, K, large = 32, 8192, 65536
+    M, N = (small, large) if kind == "shortA" else (large, small)
+    A = torch.empty((M, K), dtype=dt, device=dev)
+    B = torch.empty((K, N), dtype=dt, device=dev)
+    C = torch.empty((M, N), dtype=dt, device=dev)
+    blas.fill(A, M, K, 1, 21)
+    blas.fill(B, K, N, 1, 22)
+    blas.fill(C, M, N, 1, 23)
+    j0, nj = 4000, 96
+    band = (slice(None), slice(j0, j0 + nj)) if kind == "shortA" else (slice(j0, j0 + nj), slice(None))
+    c0 = np.ascontiguousarray(C[band].cpu().numpy())
+    C1 = C.clone()
+    blas.gemm("N", "N", A, B, C, 1.0, 1.0)
+    blas.gemm("N", "N", A, B, C1, 1.0, 1.0, algo=blas.ALGO_SKINNY, splitk=1)
+    torch.cuda.synchronize()
+    tol = TOL64 if prec == "d" else tol32(K)
+    if kind == "shortA":
+        a = np.ascontiguousarray(A.cpu().numpy())
+        b = np.ascontiguousarray(B[:, j0:j0 + nj].cpu().numpy())
+    else:
+        a = np.ascontiguousarray(A[j0:j0 + nj].cpu().numpy())
+        b = np.ascontiguousarray(B.cpu().numpy())
+    got = np.ascontiguousarray(C[band].cpu().numpy())
+    ref = orc.gemm_xp("N", "N", a, b, c0, 1.0, 1.0)
+    err, _ = orc.normwise(got, ref)
+    assert err <= tol, err
+    serr, _ = orc.normwise(np.ascontiguousarray(C.cpu().numpy()),
+                           np.ascontiguousarray(C1.cpu().numpy()).astype(np.float64))
+    assert serr <= tol, serr

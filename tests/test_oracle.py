@@ -1,0 +1,276 @@
+"""Pins the CPU oracle (oracle/oracle.c) before anything is checked against it.
+
+Each check is independent of the oracle's own formulas: exact int64 matrix
+products of the reference demos' integer patterns, hand-derived rounding cases
+that fix the chunk structure of the reference host loops, exact rational
+brute force, closed forms and numpy's own norms.
+"""
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+
+def exact_int(A, B):
+    return np.asarray(A, dtype=np.int64) @ np.asarray(B, dtype=np.int64)
+
+
+# ---- generators --------------------------------------------------------------
+
+def test_generator_logical_index_and_padding(orc):
+    A = orc.matrix(37, 23, orc.UNIFORM, 7, ld=40)
+    AT = orc.matrix(37, 23, orc.UNIFORM, 7, ld=50, trans=True)
+    assert A.shape == (37, 23) and AT.shape == (23, 37)
+    np.testing.assert_array_equal(A, AT.T)
+    assert np.isnan(A.base[:, 23:]).all() and np.isnan(AT.base[:, 37:]).all()
+    assert (A >= -1).all() and (A < 1).all()
+    A32 = orc.matrix(37, 23, orc.UNIFORM, 7, dtype=np.float32)
+    np.testing.assert_array_equal(A32, A.astype(np.float32))
+
+
+def test_generator_distributions(orc):
+    u = orc.matrix(512, 512, orc.UNIFORM, 0)
+    assert abs(u.mean()) < 0.01 and abs(u.var() - 1 / 3) < 0.01
+    z = orc.matrix(512, 512, orc.NORMAL, 2)
+    assert abs(z.mean()) < 0.01 and abs(z.std() - 1) < 0.01
+    g = orc.matrix(512, 512, orc.LOGUNIF, 3)
+    e = np.log2(np.abs(g))
+    assert e.min() >= -8 and e.max() < 8 and abs(e.mean()) < 0.05
+    assert abs((g > 0).mean() - 0.5) < 0.01
+    # different seeds differ, same seed repeats
+    assert not np.array_equal(orc.matrix(8, 8, orc.UNIFORM, 0), orc.matrix(8, 8, orc.UNIFORM, 1))
+    np.testing.assert_array_equal(orc.matrix(8, 8, orc.UNIFORM, 5), orc.matrix(8, 8, orc.UNIFORM, 5))
+
+
+def test_generator_patterns(orc):
+    A = orc.matrix(5, 13, orc.PATTERN11, 0)
+    i, j = np.meshgrid(np.arange(5), np.arange(13), indexing="ij")
+    np.testing.assert_array_equal(A, (i * 13 + j) % 11)
+
+
+# ---- known answers: the reference demos' integer matrices --------------------
+# A = (i*K + k) % 11, B = (k*N + j) % 7
+# (irr_shortA_dgemm_release_V0.0/host_code/irr_shortA_dgemm_demo.c:67,76,
+#  irr_thinB_sgemm_release_V1.2/host_code/irr_thinB_sgemm_demo.c:77,86)
+
+@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (6, 64, 512), (13, 29, 1100), (32, 48, 3072 + 700)])
+def test_kat_integer_patterns(orc, M, N, K):
+    A = orc.matrix(M, K, orc.PATTERN11, 0, ld=K + 3)
+    B = orc.matrix(K, N, orc.PATTERN7, 0, ld=N + 5)
+    P = exact_int(A, B)
+
+    C = np.zeros((M, N))
+    orc.dgemm_v11_host(A, B, C)
+    np.testing.assert_array_equal(C, P)
+
+    C = np.zeros((M, N), dtype=np.float32)
+    orc.sgemm_plain_host(A.astype(np.float32), B.astype(np.float32), C)
+    np.testing.assert_array_equal(C, P)
+
+    # sgemm V1.3.2 with the launcher's alpha = 5, beta = 3
+    # (sgemm_release_V1.3.2/device_code/sgemm_launcher.c:11-12)
+    C = np.zeros((M, N), dtype=np.float32)
+    orc.sgemm_v132_host(A.astype(np.float32), B.astype(np.float32), C, 5.0, 3.0)
+    np.testing.assert_array_equal(C, 5 * P + 3)
+
+    C = np.zeros((M, N), dtype=np.float32)
+    orc.thinB_sgemm_v12_host(A.astype(np.float32), B.astype(np.float32), C, 5.0, 3.0)
+    np.testing.assert_array_equal(C, 5 * P + 3)
+
+    C0 = orc.matrix(M, N, orc.PATTERN7, 9)
+    C = C0.copy()
+    orc.gemm_blas_host(A, B, C, 1.5, -0.5)
+    np.testing.assert_array_equal(C, 1.5 * P - 0.5 * C0)
+    C = C0.astype(np.float32)
+    orc.gemm_blas_host(A.astype(np.float32), B.astype(np.float32), C, 1.5, -0.5)
+    np.testing.assert_array_equal(C, 1.5 * P - 0.5 * C0)
+
+    for ta in "NT":
+        for tb in "NT":
+            Aop = orc.matrix(M, K, orc.PATTERN11, 0, trans=ta == "T", ld=(M if ta == "T" else K) + 1)
+            Bop = orc.matrix(K, N, orc.PATTERN7, 0, trans=tb == "T", ld=(K if tb == "T" else N) + 2)
+            R = orc.gemm_xp(ta, tb, Aop, Bop, C0, 2.0, 3.0)
+            np.testing.assert_array_equal(R, 2 * P + 3 * C0)
+            R = orc.gemm_xp(ta, tb, Aop.astype(np.float32), Bop.astype(np.float32),
+                            C0.astype(np.float32), 2.0, 3.0)
+            np.testing.assert_array_equal(R, 2 * P + 3 * C0)
+
+
+# ---- rounding cases that fix the reference's accumulation tree ---------------
+
+def _row_case(K, head, dtype):
+    """A = 1 x K row [head, 1, 1, ...], B = K x 1 ones."""
+    A = np.ones((1, K), dtype=dtype)
+    A[0, 0] = head
+    B = np.ones((K, 1), dtype=dtype)
+    return A, B
+
+
+def test_chunks_dgemm_v11(orc):
+    # k = 0 .. 511 is one chunk: 2^53 + 1 + ... rounds back to 2^53 at every
+    # step (ties-to-even); the tail chunk k = 512, 513 sums to 2 exactly, so
+    # the result is 2^53 + 2.  A chain without chunks gives 2^53; K_G = 256
+    # gives 2^53 + 256 + 2; dropping the tail gives 2^53.
+    A, B = _row_case(514, 2.0 ** 53, np.float64)
+    C = np.zeros((1, 1))
+    orc.dgemm_v11_host(A, B, C)
+    assert C[0, 0] == 2.0 ** 53 + 2
+    C = np.zeros((1, 1))
+    orc.dgemm_v11_host(A, B, C, K_G=256)
+    assert C[0, 0] == 2.0 ** 53 + 256 + 2
+    # the BLAS form keeps the same tree
+    C = np.zeros((1, 1))
+    orc.gemm_blas_host(A, B, C, 1.0, 0.0)
+    assert C[0, 0] == 2.0 ** 53 + 2
+
+
+def test_chunks_sgemm(orc):
+    # float tmp: fma runs in double, the assignment rounds to float, so
+    # 2^24 + 1 -> 2^24 inside a chunk; the tail adds 2.
+    A, B = _row_case(514, 2.0 ** 24, np.float32)
+    for fn in (orc.sgemm_plain_host,):
+        C = np.zeros((1, 1), dtype=np.float32)
+        fn(A, B, C)
+        assert C[0, 0] == 2.0 ** 24 + 2
+    C = np.zeros((1, 1), dtype=np.float32)
+    orc.sgemm_v132_host(A, B, C, 1.0, 0.0)
+    assert C[0, 0] == 2.0 ** 24 + 2
+    C = np.zeros((1, 1), dtype=np.float32)
+    orc.gemm_blas_host(A, B, C, 1.0, 0.0)
+    assert C[0, 0] == 2.0 ** 24 + 2
+
+
+def test_chunks_thinB_v12(orc):
+    # K_G = 3072 outer, K_A = 512 inner chunks.  head 2^24 then 3071 ones:
+    # inner chunk 0 (k < 512) = 2^24, inner chunks 1..5 = 512 each (exact);
+    # tmp_i adds them in order: 2^24 + 512 * 5 exactly.  The next outer chunk
+    # k = 3072 .. 3073 sums to 2; tmp_o = 2^24 + 2560 + 2.
+    A, B = _row_case(3074, 2.0 ** 24, np.float32)
+    C = np.zeros((1, 1), dtype=np.float32)
+    orc.thinB_sgemm_v12_host(A, B, C, 1.0, 0.0)
+    assert C[0, 0] == 2.0 ** 24 + 2560 + 2
+    C = np.zeros((1, 1), dtype=np.float32)
+    orc.thinB_sgemm_v12_host(A, B, C, 1.0, 0.0, K_G=3072, K_A=3072)
+    assert C[0, 0] == 2.0 ** 24 + 2  # one inner chunk: all 3071 ones lost
+
+
+def _f32(x):
+    return Fraction(float(np.float32(float(x))))
+
+
+def test_alpha_folding_sites(orc):
+    # sgemm V1.3.2 rounds alpha*A to float before the fma
+    # (sgemm_demo.c:228); thinB V1.2 rounds alpha*B (irr_thinB_sgemm_demo.c:231).
+    # With a = 1 + 2^-23, b = 1 + 2 * 2^-23, alpha = 5 the two sites give
+    # different floats; the expected values below follow from exact rational
+    # arithmetic with one rounding per C operation (the products of floats
+    # are exact in double, so the double fma adds no rounding of its own).
+    a, b, alpha = 1 + 2.0 ** -23, 1 + 2.0 ** -22, 5.0
+    A = np.array([[a]], dtype=np.float32)
+    B = np.array([[b]], dtype=np.float32)
+    fold_a = _f32(_f32(Fraction(alpha) * Fraction(a)) * Fraction(b))
+    fold_b = _f32(Fraction(a) * _f32(Fraction(alpha) * Fraction(b)))
+    assert fold_a != fold_b  # the case discriminates
+    C = np.zeros((1, 1), dtype=np.float32)
+    orc.sgemm_v132_host(A, B, C, alpha, 0.0)
+    assert Fraction(float(C[0, 0])) == fold_a
+    C = np.zeros((1, 1), dtype=np.float32)
+    orc.thinB_sgemm_v12_host(A, B, C, alpha, 0.0)
+    assert Fraction(float(C[0, 0])) == fold_b
+
+
+def test_scalar_beta_vs_blas_beta(orc):
+    # reference: C_cpu starts at 0 (sgemm_demo.c:87) and gets + beta; the BLAS
+    # form scales the input C.
+    A = np.ones((2, 3), dtype=np.float32)
+    B = np.ones((3, 2), dtype=np.float32)
+    C = np.full((2, 2), 10.0, dtype=np.float32)
+    orc.sgemm_v132_host(A, B, C, 2.0, 3.0)
+    np.testing.assert_array_equal(C, 10 + 6 + 3)
+    C = np.full((2, 2), 10.0, dtype=np.float32)
+    orc.gemm_blas_host(A, B, C, 2.0, 3.0)
+    np.testing.assert_array_equal(C, 6 + 30)
+
+
+# ---- extended-precision oracle vs exact rationals ----------------------------
+
+@pytest.mark.parametrize("ta", "NT")
+@pytest.mark.parametrize("tb", "NT")
+def test_xp_vs_fractions(orc, ta, tb):
+    rng = np.random.default_rng(11)
+    M, N, K = 5, 4, 7
+    alpha, beta = 0.7, -1.3
+    Aop = rng.standard_normal((M, K))
+    Bop = rng.standard_normal((K, N))
+    C0 = rng.standard_normal((M, N))
+    A = np.ascontiguousarray(Aop.T) if ta == "T" else Aop
+    B = np.ascontiguousarray(Bop.T) if tb == "T" else Bop
+    R = orc.gemm_xp(ta, tb, A, B, C0, alpha, beta)
+    for i in range(M):
+        for j in range(N):
+            s = sum(Fraction(Aop[i, k]) * Fraction(Bop[k, j]) for k in range(K))
+            exact = Fraction(alpha) * s + Fraction(beta) * Fraction(C0[i, j])
+            ex = float(exact)
+            assert abs(R[i, j] - ex) <= np.spacing(abs(ex)), (i, j)
+
+
+def test_xp_f32_inputs_exact_products(orc):
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((6, 9)).astype(np.float32)
+    B = rng.standard_normal((9, 5)).astype(np.float32)
+    C = rng.standard_normal((6, 5)).astype(np.float32)
+    R = orc.gemm_xp("N", "N", A, B, C, 1.0, 1.0)
+    for i in range(6):
+        for j in range(5):
+            s = sum(Fraction(float(A[i, k])) * Fraction(float(B[k, j])) for k in range(9))
+            ex = float(s + Fraction(float(C[i, j])))
+            assert abs(R[i, j] - ex) <= np.spacing(abs(ex))
+
+
+# ---- closed forms -------------------------------------------------------------
+
+def test_closed_forms(orc):
+    M, N, K = 9, 11, 37
+    C0 = orc.matrix(M, N, orc.UNIFORM, 4)
+    ones_a = orc.matrix(M, K, orc.ONES, 0)
+    ones_b = orc.matrix(K, N, orc.ONES, 0)
+    np.testing.assert_array_equal(orc.gemm_xp("N", "N", ones_a, ones_b, C0, 2.0, 0.5), 2.0 * K + 0.5 * C0)
+    A = orc.matrix(M, K, orc.UNIFORM, 5)
+    eye = np.eye(K)
+    np.testing.assert_array_equal(orc.gemm_xp("N", "N", A, eye, np.zeros((M, K)), 1.0, 0.0), A)
+    # K = 0: C = beta C
+    np.testing.assert_array_equal(orc.gemm_xp("N", "N", np.zeros((M, 0)), np.zeros((0, N)), C0, 1.0, 2.0),
+                                  2.0 * C0)
+    # rank one: u 1^T times 1 v^T = K u v^T
+    u = np.arange(1, M + 1, dtype=np.float64)[:, None]
+    v = np.arange(1, N + 1, dtype=np.float64)[None, :]
+    R = orc.gemm_xp("N", "N", u * np.ones((1, K)), np.ones((K, 1)) * v, np.zeros((M, N)), 1.0, 0.0)
+    np.testing.assert_array_equal(R, K * u * v)
+
+
+def test_normwise_matches_numpy(orc):
+    rng = np.random.default_rng(0)
+    R = rng.standard_normal((30, 20))
+    X = R + 1e-7 * rng.standard_normal((30, 20))
+    err, mx = orc.normwise(X, R)
+    assert err == pytest.approx(np.linalg.norm(X - R) / np.linalg.norm(R), rel=1e-12)
+    assert mx == pytest.approx(np.abs(X - R).max(), rel=1e-12)
+    err32, _ = orc.normwise(X.astype(np.float32), R)
+    assert err32 == pytest.approx(np.linalg.norm(X.astype(np.float32) - R) / np.linalg.norm(R), rel=1e-9)
+
+
+def test_mode_invariance_with_padding(orc):
+    # the reference harness builds explicit transposed copies and checks all
+    # four modes against one NN result (dgemm_release_V1.1/host_code/dgemm_demo.c:74,82,101-131)
+    M, N, K = 33, 17, 45
+    C0 = orc.matrix(M, N, orc.UNIFORM, 3)
+    ref = None
+    for ta in "NT":
+        for tb in "NT":
+            A = orc.matrix(M, K, orc.UNIFORM, 1, trans=ta == "T", ld=64)
+            B = orc.matrix(K, N, orc.UNIFORM, 2, trans=tb == "T", ld=64)
+            R = orc.gemm_xp(ta, tb, A, B, C0, 1.5, -0.5)
+            assert np.isfinite(R).all()
+            if ref is None:
+                ref = R
+            np.testing.assert_array_equal(R, ref)
