@@ -73,6 +73,7 @@ def _declare(L: ctypes.CDLL) -> None:
             getattr(L, name).argtypes = mode_args + [_f, _f, _vp]
     L.blasdsp_fill_f64.argtypes = [_vp, _i64, _i64, _i64, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _d, _vp]
     L.blasdsp_fill_f32.argtypes = [_vp, _i64, _i64, _i64, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _f, _vp]
+    L.blasdsp_sgemm_split_bf16.argtypes = gen + [_f, _vp, _i64, _vp, _i64, _f, _vp, _i64, ctypes.c_int, _vp]
     for extra in _EXTRA_DECLS:
         extra(L)
 
@@ -138,6 +139,17 @@ def gemm(transA: str, transB: str, A, B, C, alpha: float = 1.0, beta: float = 0.
     if raise_on_error:
         check(name, st)
     return st
+
+
+def sgemm_split_bf16(transA: str, transB: str, A, B, C, alpha: float = 1.0, beta: float = 0.0,
+                     nterms: int = 6, stream=None) -> None:
+    """Split-BF16 FP32 GEMM on torch float32 device tensors (in place on C)."""
+    M, N = C.shape
+    K = A.shape[0] if transA.upper() == "T" else A.shape[1]
+    check("blasdsp_sgemm_split_bf16",
+          lib().blasdsp_sgemm_split_bf16(transA.encode(), transB.encode(), M, N, K, alpha, A.data_ptr(), _ld(A),
+                                         B.data_ptr(), _ld(B), beta, C.data_ptr(), _ld(C), nterms,
+                                         _stream(stream)))
 
 
 def fill(X, rows: int, cols: int, dist: int, seed: int, trans: bool = False, pad: float = 0.0, stream=None):

@@ -52,6 +52,8 @@ struct SK<double> {
     {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    // f64 16x16 output: lane group g, register r hold row g + 4 r.
+    static __device__ __forceinline__ int out_row(int g, int r) { return g + 4 * r; }
 };
 
 template <>
@@ -63,6 +65,8 @@ struct SK<float> {
     {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    // f32 16x16 output: lane group g, register r hold row 4 g + r.
+    static __device__ __forceinline__ int out_row(int g, int r) { return 4 * g + r; }
 };
 
 template <typename T>
@@ -255,7 +259,8 @@ __global__ void __launch_bounds__(256) skinny_kernel(SkinnyArgs<T> a)
         mma_block<T, MT, S_KC, L_KC>(acc, b);
     }
 
-    // Store: lane (c, g), register r of tile (mt, t) is O[mt*16 + 4g + r][n0 + col_of(t, c)].
+    // Store: lane (c, g), register r of tile (mt, t) is
+    // O[mt*16 + out_row(g, r)][n0 + col_of(t, c)].
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
 #pragma unroll
@@ -264,7 +269,7 @@ __global__ void __launch_bounds__(256) skinny_kernel(SkinnyArgs<T> a)
             if (nl >= a.Nl) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int64_t ms = mt * 16 + 4 * g + r;
+                const int64_t ms = mt * 16 + SK<T>::out_row(g, r);
                 if (ms >= a.Ms) continue;
                 const T v = acc[mt][t][r];
                 if (a.splitk > 1) {

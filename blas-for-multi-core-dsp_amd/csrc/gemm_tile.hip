@@ -41,6 +41,9 @@ struct Mma<double> {
     {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    // Output layout of the f64 16x16 block: lane group g = lane / 16 and
+    // register r hold row g + 4 r (column lane % 16).
+    static __device__ __forceinline__ int out_row(int g, int r) { return g + 4 * r; }
 };
 
 template <>
@@ -51,6 +54,8 @@ struct Mma<float> {
     {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    // f32 16x16 block: lane group g, register r hold row 4 g + r.
+    static __device__ __forceinline__ int out_row(int g, int r) { return 4 * g + r; }
 };
 
 template <typename T>
@@ -222,8 +227,8 @@ __global__ void __launch_bounds__(256) tile_gemm_kernel(Gemm<T> g, int tiles_m, 
         __syncthreads();
     }
 
-    // MFMA 16x16 output layout: lane l, register r holds row 4*(l/16) + r,
-    // column l % 16 of the 16 x 16 block.
+    // MFMA 16x16 output: lane l, register r holds column l % 16 and row
+    // Mma<T>::out_row(l / 16, r) of the 16 x 16 block.
     const T alpha = g.alpha, beta = g.beta;
 #pragma unroll
     for (int i = 0; i < Cf::MT; ++i) {
@@ -232,7 +237,7 @@ __global__ void __launch_bounds__(256) tile_gemm_kernel(Gemm<T> g, int tiles_m, 
             const int64_t col = n0 + wn + j * 16 + lr;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int64_t row = m0 + wm + i * 16 + lk * 4 + r;
+                const int64_t row = m0 + wm + i * 16 + Mma<T>::out_row(lk, r);
                 if (row < g.M && col < g.N) {
                     T *p = g.C + row * g.ldc + col;
                     T v = alpha * acc[i][j][r];

@@ -39,6 +39,11 @@ hipError_t launch_skinny(const Gemm<T> &g, int splitk, T *work, hipStream_t s);
 template <typename T>
 int skinny_pick_splitk(const Gemm<T> &g, int num_cus);
 
+// Split-BF16 FP32 GEMM (gemm_split.hip).  `work` must hold
+// split_bf16_workspace_bytes(M, N, K, nterms) bytes; nterms is 3 or 6.
+size_t split_bf16_workspace_bytes(int64_t M, int64_t N, int64_t K, int nterms);
+hipError_t launch_split_bf16(const Gemm<float> &g, int nterms, void *work, hipStream_t s);
+
 // Synthetic input generator (fill.hip).
 template <typename T>
 hipError_t launch_fill(T *X, int64_t rows, int64_t cols, int64_t ld, int trans, int dist,

@@ -168,6 +168,18 @@ BLASDSP_API int blasdsp_sgemm_ex(char transA, char transB, int64_t M, int64_t N,
                      const float *B, int64_t ldb, float beta,
                      float *C, int64_t ldc, int algo, int splitk, void *stream);
 
+/* ---- precision forms (no counterpart in the reference) -------------------
+ * FP32 GEMM on the BF16 matrix cores: each FP32 operand is split into BF16
+ * pieces (hi + mid + lo for nterms = 6, hi + lo for nterms = 3) and the
+ * products hh+hm+mh+hl+lh+mm (6) or hh+hl+lh (3) are accumulated in FP32 with
+ * two-level (per-256-K-chunk) accumulation.  The pieces are staged in a
+ * library workspace (about nterms/2 * 2 bytes per element of op(A) and op(B)).
+ * Same semantics and arguments as blasdsp_sgemm; nterms must be 3 or 6. */
+BLASDSP_API int blasdsp_sgemm_split_bf16(char transA, char transB, int64_t M, int64_t N, int64_t K,
+                                         float alpha, const float *A, int64_t lda,
+                                         const float *B, int64_t ldb, float beta,
+                                         float *C, int64_t ldc, int nterms, void *stream);
+
 /* ---- synthetic inputs ----------------------------------------------------
  * Fill the logical rows x cols matrix stored at X (device memory, leading
  * dimension ld, transposed storage when trans != 0) from the counter-based
