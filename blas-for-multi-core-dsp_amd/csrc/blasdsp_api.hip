@@ -135,7 +135,11 @@ int run(const Gemm<T> &g, int algo, int splitk, hipStream_t s)
     else if (algo == BLASDSP_ALGO_SKINNY) use_skinny = small <= 64;  // larger: tile kernel
     else if (algo == BLASDSP_ALGO_AUTO) use_skinny = skinny_shape(g);
     else return BLASDSP_ERR_INVALID;
-    if (!use_skinny) return hip_status(launch_tile(g, s));
+    if (!use_skinny) {
+        // 64 x 64 tiles when 128 x 128 tiles would leave most CUs idle
+        const int64_t t128 = ((g.M + 127) / 128) * ((g.N + 127) / 128);
+        return hip_status(launch_tile(g, t128 < num_cus() ? 64 : 128, s));
+    }
 
     if (splitk <= 0) splitk = skinny_pick_splitk(g, num_cus());
     if (splitk > 256) return BLASDSP_ERR_INVALID;

@@ -20,9 +20,10 @@ struct Gemm {
     int64_t ldc;
 };
 
-// MFMA tile kernel (gemm_tile.hip): any shape, any mode.
+// MFMA tile kernel (gemm_tile.hip): any shape, any mode; bm = 64 or 128 is the
+// workgroup tile edge.
 template <typename T>
-hipError_t launch_tile(const Gemm<T> &g, hipStream_t s);
+hipError_t launch_tile(const Gemm<T> &g, int bm, hipStream_t s);
 
 // C = beta * C over M x N (beta == 0 writes zeros): K == 0 or alpha == 0.
 template <typename T>

@@ -1,0 +1,230 @@
+// tile_body.h — one C tile of the MFMA tile GEMM, shared by the per-call
+// kernel (gemm_tile.hip) and the grouped persistent kernel (gemm_grouped.hip).
+//
+// A 256-thread workgroup computes one BM x BM tile of C = alpha op(A) op(B) +
+// beta C with v_mfma_f64_16x16x4_f64 (FP64) or v_mfma_f32_16x16x4_f32 (FP32):
+//   * four waves in a 2 x 2 grid, each owning a BM/2 x BM/2 sub-tile made of
+//     (BM/32)^2 MFMA 16x16 accumulators;
+//   * op(A) / op(B) tiles of BK = 128 bytes of K are staged global -> LDS with
+//     16-byte vector loads along whichever dimension is contiguous in HBM, and
+//     the LDS copy keeps that orientation; the MFMA fragment reads adapt to it,
+//     so the transpose modes need no transpose pass and no transposing writes;
+//   * LDS rows are padded (k-contiguous rows by 16 bytes, m/n-contiguous rows
+//     by 16 elements) so that fragment reads are bank-conflict-free per
+//     half-wave;
+//   * two LDS stages: global loads for K tile k+1 are in flight while the
+//     MFMAs consume tile k; one barrier per K tile;
+//   * epilogue C = alpha * acc + beta * C fused into the store (C is not read
+//     when beta == 0).
+#pragma once
+
+#include "internal.h"
+
+namespace blasdsp {
+namespace tile {
+
+template <typename T>
+struct Mma;
+
+template <>
+struct Mma<double> {
+    typedef double acc_t __attribute__((ext_vector_type(4)));
+    typedef double vec_t __attribute__((ext_vector_type(2)));
+    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // Output of the f64 16x16 block: lane group g = lane / 16 and register r
+    // hold row g + 4 r, column lane % 16.
+    static __device__ __forceinline__ int out_row(int g, int r) { return g + 4 * r; }
+};
+
+template <>
+struct Mma<float> {
+    typedef float acc_t __attribute__((ext_vector_type(4)));
+    typedef float vec_t __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // f32 16x16 block: lane group g, register r hold row 4 g + r.
+    static __device__ __forceinline__ int out_row(int g, int r) { return 4 * g + r; }
+};
+
+template <typename T, int BM_>
+struct Cfg {
+    static constexpr int BM = BM_;
+    static constexpr int BK = 128 / (int)sizeof(T);  // one 128-byte row of K
+    static constexpr int V = 16 / (int)sizeof(T);    // elements per 16-byte vector
+    static constexpr int SK = BK + V;                // k-contiguous LDS row stride
+    static constexpr int SMN = BM + 16;              // m/n-contiguous LDS row stride
+    static constexpr int BUF = (BM * SK > BK * SMN) ? BM * SK : BK * SMN;  // per operand stage
+    static constexpr int THREADS = 256;
+    static constexpr int WM = BM / 2;                // wave sub-tile edge
+    static constexpr int MT = WM / 16;               // MFMA tiles per wave edge
+    static constexpr int LDS_ELEMS = 2 * 2 * BUF;    // [stage][A, B]
+};
+
+// Global -> register -> LDS staging of one operand tile.  KC: the stored tile
+// is k-contiguous (BM rows of BK elements); otherwise m/n-contiguous (BK rows
+// of BM elements).
+template <typename T, int BM, bool KC>
+struct Stager {
+    using Cf = Cfg<T, BM>;
+    using vec_t = typename Mma<T>::vec_t;
+    static constexpr int TR = KC ? BM : Cf::BK;
+    static constexpr int TC = KC ? Cf::BK : BM;
+    static constexpr int CPR = TC / Cf::V;
+    static constexpr int NCH = TR * CPR / Cf::THREADS;
+    static constexpr int S = KC ? Cf::SK : Cf::SMN;
+    static_assert(TR * CPR % Cf::THREADS == 0, "whole vectors per thread");
+
+    vec_t r[NCH];
+
+    // X: stored matrix with R rows, Cn columns, leading dimension ld; the tile
+    // starts at stored (r0, c0).  Out-of-range elements read as zero.
+    __device__ __forceinline__ void load(const T *__restrict__ X, int64_t ld, int64_t R, int64_t Cn,
+                                         int64_t r0, int64_t c0, bool vec, int tid)
+    {
+        const bool full = vec && (r0 + TR <= R) && (c0 + TC <= Cn);
+        if (full) {
+#pragma unroll
+            for (int i = 0; i < NCH; ++i) {
+                const int c = tid + i * Cf::THREADS;
+                const int row = c / CPR, col = (c % CPR) * Cf::V;
+                r[i] = *reinterpret_cast<const vec_t *>(X + (r0 + row) * ld + c0 + col);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < NCH; ++i) {
+                const int c = tid + i * Cf::THREADS;
+                const int row = c / CPR, col = (c % CPR) * Cf::V;
+                const int64_t gr = r0 + row, gc = c0 + col;
+                if (vec && gr < R && gc + Cf::V <= Cn) {
+                    r[i] = *reinterpret_cast<const vec_t *>(X + gr * ld + gc);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < Cf::V; ++e)
+                        r[i][e] = (gr < R && gc + e < Cn) ? X[gr * ld + gc + e] : T(0);
+                }
+            }
+        }
+    }
+
+    __device__ __forceinline__ void store(T *s, int tid) const
+    {
+#pragma unroll
+        for (int i = 0; i < NCH; ++i) {
+            const int c = tid + i * Cf::THREADS;
+            const int row = c / CPR, col = (c % CPR) * Cf::V;
+            *reinterpret_cast<vec_t *>(s + row * S + col) = r[i];
+        }
+    }
+};
+
+// One BM x BM tile of C at (m0, n0).  lds: Cfg<T, BM>::LDS_ELEMS elements.
+// Every wave of the workgroup must call it (it contains barriers).
+template <typename T, int BM, bool TA, bool TB>
+__device__ __forceinline__ void tile_body(const Gemm<T> &g, int64_t m0, int64_t n0, bool a_vec,
+                                          bool b_vec, T *lds)
+{
+    using Cf = Cfg<T, BM>;
+    using acc_t = typename Mma<T>::acc_t;
+    constexpr int BK = Cf::BK, SK = Cf::SK, SMN = Cf::SMN, MT = Cf::MT, BUF = Cf::BUF;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int wm = (wave >> 1) * Cf::WM, wn = (wave & 1) * Cf::WM;
+
+    // Stored geometry: A is K x M when transposed, else M x K; B is N x K when
+    // transposed, else K x N.
+    const int64_t AR = TA ? g.K : g.M, AC = TA ? g.M : g.K;
+    const int64_t BR = TB ? g.N : g.K, BC = TB ? g.K : g.N;
+
+    Stager<T, BM, !TA> sa;
+    Stager<T, BM, TB> sb;
+    auto load_tiles = [&](int64_t k0) {
+        if (TA) sa.load(g.A, g.lda, AR, AC, k0, m0, a_vec, tid);
+        else    sa.load(g.A, g.lda, AR, AC, m0, k0, a_vec, tid);
+        if (TB) sb.load(g.B, g.ldb, BR, BC, n0, k0, b_vec, tid);
+        else    sb.load(g.B, g.ldb, BR, BC, k0, n0, b_vec, tid);
+    };
+    auto stage = [&](int st, int op) { return lds + (st * 2 + op) * BUF; };
+
+    acc_t acc[MT][MT];
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int j = 0; j < MT; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+    const int nk = (int)((g.K + BK - 1) / BK);
+    load_tiles(0);
+    sa.store(stage(0, 0), tid);
+    sb.store(stage(0, 1), tid);
+    __syncthreads();
+
+    for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        const bool more = kt + 1 < nk;
+        if (more) load_tiles((int64_t)(kt + 1) * BK);
+
+        const T *As = stage(cur, 0);
+        const T *Bs = stage(cur, 1);
+#pragma unroll
+        for (int ks = 0; ks < BK / 4; ++ks) {
+            const int k = ks * 4 + lk;
+            T a[MT], b[MT];
+#pragma unroll
+            for (int i = 0; i < MT; ++i) {
+                const int m = wm + i * 16 + lr;
+                a[i] = TA ? As[k * SMN + m] : As[m * SK + k];
+            }
+#pragma unroll
+            for (int j = 0; j < MT; ++j) {
+                const int n = wn + j * 16 + lr;
+                b[j] = TB ? Bs[n * SK + k] : Bs[k * SMN + n];
+            }
+#pragma unroll
+            for (int i = 0; i < MT; ++i)
+#pragma unroll
+                for (int j = 0; j < MT; ++j) acc[i][j] = Mma<T>::mma(a[i], b[j], acc[i][j]);
+        }
+
+        if (more) {
+            sa.store(stage(cur ^ 1, 0), tid);
+            sb.store(stage(cur ^ 1, 1), tid);
+        }
+        __syncthreads();
+    }
+
+    const T alpha = g.alpha, beta = g.beta;
+#pragma unroll
+    for (int i = 0; i < MT; ++i) {
+#pragma unroll
+        for (int j = 0; j < MT; ++j) {
+            const int64_t col = n0 + wn + j * 16 + lr;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t row = m0 + wm + i * 16 + Mma<T>::out_row(lk, r);
+                if (row < g.M && col < g.N) {
+                    T *p = g.C + row * g.ldc + col;
+                    T v = alpha * acc[i][j][r];
+                    if (beta != T(0)) v += beta * *p;
+                    *p = v;
+                }
+            }
+        }
+    }
+}
+
+// Vector loads are used when the base is 16-byte aligned and every row starts
+// on a 16-byte boundary.
+template <typename T>
+inline bool vec_ok(const T *p, int64_t ld)
+{
+    return ((uintptr_t)p % 16 == 0) && (ld % (16 / (int64_t)sizeof(T)) == 0);
+}
+
+}  // namespace tile
+}  // namespace blasdsp
