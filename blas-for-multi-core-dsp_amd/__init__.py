@@ -37,6 +37,19 @@ _f = ctypes.c_float
 _lib = None
 
 
+FLOAT32, FLOAT64 = 0, 1
+
+
+class GemmDesc(ctypes.Structure):
+    """blasdsp_gemm_desc (include/blasdsp.h)."""
+    _fields_ = [("dtype", ctypes.c_int), ("transA", ctypes.c_char), ("transB", ctypes.c_char),
+                ("M", ctypes.c_int64), ("N", ctypes.c_int64), ("K", ctypes.c_int64),
+                ("alpha", ctypes.c_double), ("beta", ctypes.c_double),
+                ("A", ctypes.c_void_p), ("lda", ctypes.c_int64),
+                ("B", ctypes.c_void_p), ("ldb", ctypes.c_int64),
+                ("C", ctypes.c_void_p), ("ldc", ctypes.c_int64)]
+
+
 class BlasdspError(RuntimeError):
     def __init__(self, fn: str, status: int):
         super().__init__(f"{fn} returned {status}")
@@ -74,6 +87,10 @@ def _declare(L: ctypes.CDLL) -> None:
     L.blasdsp_fill_f64.argtypes = [_vp, _i64, _i64, _i64, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _d, _vp]
     L.blasdsp_fill_f32.argtypes = [_vp, _i64, _i64, _i64, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _f, _vp]
     L.blasdsp_sgemm_split_bf16.argtypes = gen + [_f, _vp, _i64, _vp, _i64, _f, _vp, _i64, ctypes.c_int, _vp]
+    L.blasdsp_gemm_grouped.argtypes = [ctypes.POINTER(GemmDesc), ctypes.c_int, _vp]
+    L.blasdsp_group_plan_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+    L.blasdsp_group_plan_run.argtypes = [ctypes.c_void_p, ctypes.POINTER(GemmDesc), ctypes.c_int, _vp]
+    L.blasdsp_group_plan_destroy.argtypes = [ctypes.c_void_p]
     for extra in _EXTRA_DECLS:
         extra(L)
 
@@ -150,6 +167,46 @@ def sgemm_split_bf16(transA: str, transB: str, A, B, C, alpha: float = 1.0, beta
           lib().blasdsp_sgemm_split_bf16(transA.encode(), transB.encode(), M, N, K, alpha, A.data_ptr(), _ld(A),
                                          B.data_ptr(), _ld(B), beta, C.data_ptr(), _ld(C), nterms,
                                          _stream(stream)))
+
+
+def desc(transA: str, transB: str, A, B, C, alpha: float = 1.0, beta: float = 0.0) -> GemmDesc:
+    """Descriptor of C = alpha op(A) op(B) + beta C over torch device tensors."""
+    import torch
+
+    M, N = C.shape
+    K = A.shape[0] if transA.upper() == "T" else A.shape[1]
+    dt = FLOAT64 if C.dtype == torch.float64 else FLOAT32
+    return GemmDesc(dt, transA.encode(), transB.encode(), M, N, K, alpha, beta, A.data_ptr(), _ld(A),
+                    B.data_ptr(), _ld(B), C.data_ptr(), _ld(C))
+
+
+def gemm_grouped(descs, stream=None) -> None:
+    """All problems of `descs` (GemmDesc list) in one grouped launch."""
+    arr = (GemmDesc * len(descs))(*descs)
+    check("blasdsp_gemm_grouped", lib().blasdsp_gemm_grouped(arr, len(descs), _stream(stream)))
+
+
+class GroupPlan:
+    """blasdsp_group_plan: a reusable hipGraph of the grouped kernels."""
+
+    def __init__(self, max_problems: int):
+        self._p = ctypes.c_void_p()
+        check("blasdsp_group_plan_create", lib().blasdsp_group_plan_create(max_problems, ctypes.byref(self._p)))
+
+    def run(self, descs, stream=None) -> None:
+        arr = (GemmDesc * len(descs))(*descs)
+        check("blasdsp_group_plan_run", lib().blasdsp_group_plan_run(self._p, arr, len(descs), _stream(stream)))
+
+    def close(self) -> None:
+        if self._p:
+            check("blasdsp_group_plan_destroy", lib().blasdsp_group_plan_destroy(self._p))
+            self._p = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def fill(X, rows: int, cols: int, dist: int, seed: int, trans: bool = False, pad: float = 0.0, stream=None):

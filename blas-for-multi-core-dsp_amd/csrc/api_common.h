@@ -1,0 +1,43 @@
+// api_common.h — helpers shared by the C-ABI translation units.
+#pragma once
+
+#include "../../include/blasdsp.h"
+#include "internal.h"
+
+namespace blasdsp {
+
+extern thread_local int g_last_hip_error;
+
+// hipError_t -> BLASDSP_* status (records the HIP code for blasdsp_last_hip_error).
+int hip_status(hipError_t e);
+// Compute units of the current device (cached).
+int num_cus();
+// Grow-only scratch buffer owned by the library, one per (device, stream).
+hipError_t workspace(hipStream_t s, size_t bytes, void **out);
+bool parse_trans(char t, bool &out);
+
+// Argument checks shared by every entry point (reference shape limits,
+// dgemm_release_V1.1/readme.txt:11-15, are not carried over: any M, N, K >= 0
+// and any leading dimension covering a row are accepted).
+template <typename T>
+int check(Gemm<T> &g, char transA, char transB, int64_t M, int64_t N, int64_t K, T alpha,
+          const T *A, int64_t lda, const T *B, int64_t ldb, T beta, T *C, int64_t ldc)
+{
+    if (!parse_trans(transA, g.ta) || !parse_trans(transB, g.tb)) return BLASDSP_ERR_INVALID;
+    if (M < 0 || N < 0 || K < 0) return BLASDSP_ERR_INVALID;
+    const int64_t a_cols = g.ta ? M : K, b_cols = g.tb ? K : N;
+    if (lda < (a_cols > 1 ? a_cols : 1)) return BLASDSP_ERR_INVALID;
+    if (ldb < (b_cols > 1 ? b_cols : 1)) return BLASDSP_ERR_INVALID;
+    if (ldc < (N > 1 ? N : 1)) return BLASDSP_ERR_INVALID;
+    if (M > 0 && N > 0) {
+        if (!C) return BLASDSP_ERR_INVALID;
+        if (K > 0 && alpha != T(0) && (!A || !B)) return BLASDSP_ERR_INVALID;
+    }
+    g.M = M; g.N = N; g.K = K;
+    g.alpha = alpha; g.beta = beta;
+    g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
+    return BLASDSP_SUCCESS;
+}
+
+
+}  // namespace blasdsp

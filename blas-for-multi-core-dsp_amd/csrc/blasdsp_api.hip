@@ -17,10 +17,9 @@
 #include <utility>
 
 #include "../../include/blasdsp.h"
-#include "internal.h"
+#include "api_common.h"
 
 namespace blasdsp {
-namespace {
 
 thread_local int g_last_hip_error = 0;
 
@@ -37,12 +36,14 @@ int hip_status(hipError_t e)
 // stream are ordered, so reuse within a stream is safe; distinct streams never
 // share a buffer.  Growing frees the old buffer with hipFree, which waits for
 // the device, so in-flight users of the old buffer finish first.
+namespace {
 struct Workspace {
     void *ptr = nullptr;
     size_t bytes = 0;
 };
 std::mutex g_ws_mutex;
 std::map<std::pair<int, hipStream_t>, Workspace> g_ws;
+}  // namespace
 
 hipError_t workspace(hipStream_t s, size_t bytes, void **out)
 {
@@ -93,28 +94,7 @@ bool parse_trans(char t, bool &out)
     }
 }
 
-// Argument checks shared by every entry point (reference shape limits,
-// dgemm_release_V1.1/readme.txt:11-15, are not carried over: any M, N, K >= 0
-// and any leading dimension covering a row are accepted).
-template <typename T>
-int check(Gemm<T> &g, char transA, char transB, int64_t M, int64_t N, int64_t K, T alpha,
-          const T *A, int64_t lda, const T *B, int64_t ldb, T beta, T *C, int64_t ldc)
-{
-    if (!parse_trans(transA, g.ta) || !parse_trans(transB, g.tb)) return BLASDSP_ERR_INVALID;
-    if (M < 0 || N < 0 || K < 0) return BLASDSP_ERR_INVALID;
-    const int64_t a_cols = g.ta ? M : K, b_cols = g.tb ? K : N;
-    if (lda < (a_cols > 1 ? a_cols : 1)) return BLASDSP_ERR_INVALID;
-    if (ldb < (b_cols > 1 ? b_cols : 1)) return BLASDSP_ERR_INVALID;
-    if (ldc < (N > 1 ? N : 1)) return BLASDSP_ERR_INVALID;
-    if (M > 0 && N > 0) {
-        if (!C) return BLASDSP_ERR_INVALID;
-        if (K > 0 && alpha != T(0) && (!A || !B)) return BLASDSP_ERR_INVALID;
-    }
-    g.M = M; g.N = N; g.K = K;
-    g.alpha = alpha; g.beta = beta;
-    g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
-    return BLASDSP_SUCCESS;
-}
+namespace {
 
 template <typename T>
 bool skinny_shape(const Gemm<T> &g)

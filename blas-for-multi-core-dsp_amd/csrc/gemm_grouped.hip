@@ -1,0 +1,135 @@
+// gemm_grouped.hip — many GEMMs of uneven shapes and transpose modes in one
+// persistent launch per precision.
+//
+// No counterpart in the reference (which runs one GEMM per hthread_group_exec,
+// dgemm_release_V1.1/host_code/dgemm_demo.c:151).  The host flattens a group
+// of problems into a list of C tiles: problem p owns tiles
+// [tile_begin(p), tile_begin(p) + tiles_m * tiles_n).  A fixed grid of
+// workgroups (the tile kernel occupancy times the CU count) strides over it;
+// each workgroup finds its problem by binary search over tile_begin and runs
+// the same tile body as the per-call kernel (tile_body.h), so a problem's C is
+// bit-identical alone and in a group.  Problems are sorted by K (the cost of
+// one of their tiles) in decreasing order, so the strided assignment deals the
+// expensive tiles out first.  Each problem picks a 64 x 64 or 128 x 128 tile;
+// there is one kernel per (precision, tile edge), each with the register
+// budget and LDS of its tile size (FP64 128 tiles also one kernel per
+// transpose mode), and the kernels are independent graph nodes that run side
+// by side on the device.
+//
+// The problem list lives in device memory and the grid does not depend on it,
+// so one instantiated hipGraph (built from explicit kernel nodes, no stream
+// capture and no memset nodes) replays any group: the host uploads the list
+// and launches the graph.
+#include <stdlib.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "grouped.h"
+#include "tile_body.h"
+
+namespace blasdsp {
+namespace {
+
+using tile::Cfg;
+
+template <typename T, int BM, bool TA, bool TB>
+__device__ __forceinline__ void run_tile(const GroupProblem &P, int64_t local, T *lds)
+{
+    Gemm<T> g;
+    g.ta = TA; g.tb = TB;
+    g.M = P.M; g.N = P.N; g.K = P.K;
+    g.alpha = (T)P.alpha; g.beta = (T)P.beta;
+    g.A = (const T *)P.A; g.lda = P.lda;
+    g.B = (const T *)P.B; g.ldb = P.ldb;
+    g.C = (T *)P.C; g.ldc = P.ldc;
+    const int64_t tm = local / P.tiles_n, tn = local % P.tiles_n;
+    tile::tile_body<T, BM, TA, TB>(g, tm * BM, tn * BM, P.a_vec, P.b_vec, lds);
+}
+
+// MODE < 0: the problem's transpose mode is read per tile; else fixed.
+template <typename T, int BM, int MODE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Cfg<T, BM>::WPE, Cfg<T, BM>::WPE)))
+grouped_kernel(const GroupHeader *__restrict__ hdr, const GroupProblem *__restrict__ probs)
+{
+    __shared__ __attribute__((aligned(16))) T lds[Cfg<T, BM>::LDS_ELEMS];
+    const int w = group_list(sizeof(T) == 8, BM, MODE < 0 ? 0 : MODE);
+    const int count = hdr->count[w];
+    const int64_t total = hdr->total_tiles[w];
+    probs += hdr->first[w];
+    for (int64_t t = blockIdx.x; t < total; t += gridDim.x) {
+        int lo = 0, hi = count - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (probs[mid].tile_begin <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const GroupProblem &P = probs[lo];
+        const int64_t local = t - P.tile_begin;
+        switch (MODE < 0 ? ((P.ta << 1) | P.tb) : MODE) {
+        case 0: run_tile<T, BM, false, false>(P, local, lds); break;
+        case 1: run_tile<T, BM, false, true>(P, local, lds); break;
+        case 2: run_tile<T, BM, true, false>(P, local, lds); break;
+        default: run_tile<T, BM, true, true>(P, local, lds); break;
+        }
+        __syncthreads();  // the next tile reuses the LDS stages
+    }
+}
+
+}  // namespace
+
+int grouped_grid(int w, int num_cus)
+{
+    const int cus = num_cus > 0 ? num_cus : 256;
+    int wpe;
+    switch (w) {
+    case 0: wpe = Cfg<float, 128>::WPE; break;
+    case 1: wpe = Cfg<float, 64>::WPE; break;
+    case 2: wpe = Cfg<double, 64>::WPE; break;
+    default: wpe = Cfg<double, 128>::WPE; break;
+    }
+    return cus * wpe;  // 4 waves per workgroup, 4 SIMDs per CU
+}
+
+void *grouped_kernel_ptr(int w)
+{
+    switch (w) {
+    case 0: return reinterpret_cast<void *>(&grouped_kernel<float, 128, -1>);
+    case 1: return reinterpret_cast<void *>(&grouped_kernel<float, 64, -1>);
+    case 2: return reinterpret_cast<void *>(&grouped_kernel<double, 64, -1>);
+    case 3: return reinterpret_cast<void *>(&grouped_kernel<double, 128, 0>);
+    case 4: return reinterpret_cast<void *>(&grouped_kernel<double, 128, 1>);
+    case 5: return reinterpret_cast<void *>(&grouped_kernel<double, 128, 2>);
+    default: return reinterpret_cast<void *>(&grouped_kernel<double, 128, 3>);
+    }
+}
+
+int64_t build_group(const std::vector<GroupProblem> (&lists)[kGroupLists], GroupHeader *hdr,
+                    GroupProblem *out, int capacity)
+{
+    int pos = 0;
+    int64_t tiles_all = 0;
+    for (int w = 0; w < kGroupLists; ++w) {
+        std::vector<GroupProblem> v = lists[w];
+        std::stable_sort(v.begin(), v.end(),
+                         [](const GroupProblem &a, const GroupProblem &b) { return a.K > b.K; });
+        hdr->first[w] = pos;
+        int64_t tiles = 0;
+        for (GroupProblem p : v) {
+            if (pos >= capacity) return -1;
+            const int64_t bm = p.bm;
+            const int64_t tm = (p.M + bm - 1) / bm;
+            p.tiles_n = (p.N + bm - 1) / bm;
+            p.tile_begin = tiles;
+            tiles += tm * p.tiles_n;
+            out[pos++] = p;
+        }
+        hdr->count[w] = (int)v.size();
+        hdr->total_tiles[w] = tiles;
+        tiles_all += tiles;
+    }
+    return tiles_all;
+}
+
+}  // namespace blasdsp

@@ -53,8 +53,8 @@ __device__ __forceinline__ void tile_of_block(int tiles_m, int tiles_n, int &tm,
 }
 
 template <typename T, int BM, bool TA, bool TB>
-__global__ void __launch_bounds__(256) tile_gemm_kernel(Gemm<T> g, int tiles_m, int tiles_n,
-                                                        bool a_vec, bool b_vec)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Cfg<T, BM>::WPE, Cfg<T, BM>::WPE)))
+tile_gemm_kernel(Gemm<T> g, int tiles_m, int tiles_n, bool a_vec, bool b_vec)
 {
     __shared__ __attribute__((aligned(16))) T lds[Cfg<T, BM>::LDS_ELEMS];
     int tm, tn;

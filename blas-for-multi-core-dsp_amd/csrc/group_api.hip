@@ -1,0 +1,259 @@
+// group_api.hip — C ABI of the grouped persistent GEMM and its hipGraph plan.
+//
+// A plan owns the device problem list, a pinned staging copy of it and an
+// instantiated graph of two kernel nodes (the FP32 and FP64 grouped kernels,
+// in that order).  Running a plan validates the descriptors, builds the
+// sorted tile list in the staging buffer, uploads it with one async copy on
+// the caller's stream and replays the graph on that stream.  The staging
+// buffer is reused only after the previous upload has completed (an event
+// recorded after the copy).
+#include <stdlib.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+#include <new>
+#include <utility>
+#include <vector>
+
+#include "api_common.h"
+#include "grouped.h"
+#include "tile_body.h"
+
+struct blasdsp_group_plan {
+    int device = 0;
+    int capacity = 0;
+    size_t list_bytes = 0;        // header slot + capacity problems
+    char *d_list = nullptr;       // device: [GroupHeader | pad][GroupProblem x capacity]
+    char *h_list = nullptr;       // pinned staging, same layout
+    hipEvent_t uploaded = nullptr;  // the staging copy has been read
+    hipEvent_t done = nullptr;      // the last graph replay has finished
+    bool pending_upload = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraphNode_t nodes[blasdsp::kGroupLists] = {};
+    int full_grid[blasdsp::kGroupLists] = {};
+    int cur_grid[blasdsp::kGroupLists] = {};
+    bool enabled[blasdsp::kGroupLists] = {};
+};
+
+namespace blasdsp {
+namespace {
+
+constexpr size_t kHeaderSlot = 256;
+
+GroupHeader *hdr_of(char *base) { return reinterpret_cast<GroupHeader *>(base); }
+GroupProblem *probs_of(char *base) { return reinterpret_cast<GroupProblem *>(base + kHeaderSlot); }
+
+// Kernel node parameters of list w.  The argument values are copied when the
+// node is added or updated, so the locals behind kernelParams may go away.
+struct NodeArgs {
+    const GroupHeader *hdr;
+    const GroupProblem *probs;
+    void *ptrs[2];
+};
+
+hipKernelNodeParams node_params(blasdsp_group_plan *p, int w, int grid)
+{
+    static thread_local NodeArgs a;
+    a.hdr = hdr_of(p->d_list);
+    a.probs = probs_of(p->d_list);
+    a.ptrs[0] = (void *)&a.hdr;
+    a.ptrs[1] = (void *)&a.probs;
+    hipKernelNodeParams kp{};
+    kp.func = grouped_kernel_ptr(w);
+    kp.gridDim = dim3(grid);
+    kp.blockDim = dim3(256);
+    kp.sharedMemBytes = 0;
+    kp.kernelParams = a.ptrs;
+    kp.extra = nullptr;
+    return kp;
+}
+
+void destroy_plan(blasdsp_group_plan *p)
+{
+    if (!p) return;
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->uploaded) (void)hipEventDestroy(p->uploaded);
+    if (p->done) (void)hipEventDestroy(p->done);
+    if (p->h_list) (void)hipHostFree(p->h_list);
+    if (p->d_list) (void)hipFree(p->d_list);
+    delete p;
+}
+
+int create_plan(int capacity, blasdsp_group_plan **out)
+{
+    if (capacity <= 0 || !out) return BLASDSP_ERR_INVALID;
+    blasdsp_group_plan *p = new (std::nothrow) blasdsp_group_plan();
+    if (!p) return BLASDSP_ERR_NOMEM;
+    hipError_t e = hipGetDevice(&p->device);
+    p->capacity = capacity;
+    p->list_bytes = kHeaderSlot + (size_t)capacity * sizeof(GroupProblem);
+    if (e == hipSuccess) e = hipMalloc(&p->d_list, p->list_bytes);
+    if (e == hipSuccess) e = hipHostMalloc(&p->h_list, p->list_bytes, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->uploaded, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipGraphCreate(&p->graph, 0);
+    if (e != hipSuccess) {
+        destroy_plan(p);
+        return hip_status(e);
+    }
+    memset(p->h_list, 0, p->list_bytes);
+    // One kernel node per problem list over one device list; with no edges
+    // between them they may run concurrently.  Nodes of empty lists are
+    // disabled at run time and each node's grid is trimmed to its tile count.
+    const int cus = num_cus();
+    for (int w = 0; w < kGroupLists; ++w) {
+        p->full_grid[w] = p->cur_grid[w] = grouped_grid(w, cus);
+        p->enabled[w] = true;
+        hipKernelNodeParams kp = node_params(p, w, p->full_grid[w]);
+        e = hipGraphAddKernelNode(&p->nodes[w], p->graph, nullptr, 0, &kp);
+        if (e != hipSuccess) {
+            destroy_plan(p);
+            return hip_status(e);
+        }
+    }
+    e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        destroy_plan(p);
+        return hip_status(e);
+    }
+    *out = p;
+    return BLASDSP_SUCCESS;
+}
+
+template <typename T>
+int to_problem(const blasdsp_gemm_desc &d, GroupProblem &p, bool &skip)
+{
+    Gemm<T> g;
+    int st = check<T>(g, d.transA, d.transB, d.M, d.N, d.K, (T)d.alpha, (const T *)d.A, d.lda,
+                      (const T *)d.B, d.ldb, (T)d.beta, (T *)d.C, d.ldc);
+    if (st != BLASDSP_SUCCESS) return st;
+    skip = (d.M == 0 || d.N == 0);
+    p = GroupProblem{};
+    p.A = d.A; p.B = d.B; p.C = d.C;
+    p.M = d.M; p.N = d.N;
+    p.K = ((T)d.alpha == T(0)) ? 0 : d.K;  // alpha == 0: C = beta C, A and B not read
+    p.lda = d.lda; p.ldb = d.ldb; p.ldc = d.ldc;
+    p.alpha = d.alpha; p.beta = d.beta;
+    p.ta = g.ta; p.tb = g.tb;
+    p.a_vec = tile::vec_ok((const T *)d.A, d.lda);
+    p.b_vec = tile::vec_ok((const T *)d.B, d.ldb);
+    p.bm = (d.M <= 96 || d.N <= 96) ? 64 : 128;
+    return BLASDSP_SUCCESS;
+}
+
+int run_plan(blasdsp_group_plan *p, const blasdsp_gemm_desc *descs, int count, hipStream_t s)
+{
+    if (!p || count < 0 || (count > 0 && !descs)) return BLASDSP_ERR_INVALID;
+    if (count > p->capacity) return BLASDSP_ERR_INVALID;
+    std::vector<GroupProblem> lists[kGroupLists];
+    int used = 0;
+    for (int i = 0; i < count; ++i) {
+        GroupProblem gp;
+        bool skip = false;
+        int st;
+        const bool f64 = descs[i].dtype == BLASDSP_FLOAT64;
+        if (descs[i].dtype == BLASDSP_FLOAT32) st = to_problem<float>(descs[i], gp, skip);
+        else if (f64) st = to_problem<double>(descs[i], gp, skip);
+        else st = BLASDSP_ERR_INVALID;
+        if (st != BLASDSP_SUCCESS) return st;
+        if (!skip) {
+            lists[group_list(f64, gp.bm, (gp.ta << 1) | gp.tb)].push_back(gp);
+            ++used;
+        }
+    }
+    // The staging buffer may still be read by the previous upload.
+    if (p->pending_upload) {
+        hipError_t e = hipEventSynchronize(p->uploaded);
+        if (e != hipSuccess) return hip_status(e);
+        p->pending_upload = false;
+    }
+    GroupHeader *h = hdr_of(p->h_list);
+    if (build_group(lists, h, probs_of(p->h_list), p->capacity) < 0) return BLASDSP_ERR_INVALID;
+    const size_t bytes = kHeaderSlot + (size_t)used * sizeof(GroupProblem);
+    // Enable the nodes with work, each with no more workgroups than tiles.
+    for (int w = 0; w < kGroupLists; ++w) {
+        const int64_t tiles = h->total_tiles[w];
+        const bool on = tiles > 0;
+        const int grid = (int)(tiles < p->full_grid[w] ? (tiles > 0 ? tiles : 1) : p->full_grid[w]);
+        hipError_t e = hipSuccess;
+        if (on && grid != p->cur_grid[w]) {
+            hipKernelNodeParams kp = node_params(p, w, grid);
+            e = hipGraphExecKernelNodeSetParams(p->exec, p->nodes[w], &kp);
+            if (e == hipSuccess) p->cur_grid[w] = grid;
+        }
+        if (e == hipSuccess && on != p->enabled[w]) {
+            e = hipGraphNodeSetEnabled(p->exec, p->nodes[w], on ? 1 : 0);
+            if (e == hipSuccess) p->enabled[w] = on;
+        }
+        if (e != hipSuccess) return hip_status(e);
+    }
+    hipError_t e = hipMemcpyAsync(p->d_list, p->h_list, bytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipEventRecord(p->uploaded, s);
+    if (e != hipSuccess) return hip_status(e);
+    p->pending_upload = true;
+    e = hipGraphLaunch(p->exec, s);
+    if (e == hipSuccess) e = hipEventRecord(p->done, s);
+    return hip_status(e);
+}
+
+// Plans behind blasdsp_gemm_grouped: one per (device, stream), grown on demand.
+std::mutex g_plan_mutex;
+std::map<std::pair<int, hipStream_t>, blasdsp_group_plan *> g_plans;
+
+}  // namespace
+}  // namespace blasdsp
+
+extern "C" {
+
+int blasdsp_group_plan_create(int max_problems, blasdsp_group_plan **plan)
+{
+    return blasdsp::create_plan(max_problems, plan);
+}
+
+int blasdsp_group_plan_run(blasdsp_group_plan *plan, const blasdsp_gemm_desc *descs, int count,
+                           void *stream)
+{
+    return blasdsp::run_plan(plan, descs, count, (hipStream_t)stream);
+}
+
+int blasdsp_group_plan_destroy(blasdsp_group_plan *plan)
+{
+    if (!plan) return BLASDSP_ERR_INVALID;
+    (void)hipEventSynchronize(plan->done);  // never recorded: returns at once
+    blasdsp::destroy_plan(plan);
+    return BLASDSP_SUCCESS;
+}
+
+int blasdsp_gemm_grouped(const blasdsp_gemm_desc *descs, int count, void *stream)
+{
+    if (count < 0 || (count > 0 && !descs)) return BLASDSP_ERR_INVALID;
+    if (count == 0) return BLASDSP_SUCCESS;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return blasdsp::hip_status(e);
+    hipStream_t s = (hipStream_t)stream;
+    std::lock_guard<std::mutex> lock(blasdsp::g_plan_mutex);
+    blasdsp_group_plan *&p = blasdsp::g_plans[std::make_pair(dev, s)];
+    if (p && p->capacity < count) {
+        // the previous graph may still be running on s
+        e = hipEventSynchronize(p->done);
+        if (e != hipSuccess) return blasdsp::hip_status(e);
+        blasdsp::destroy_plan(p);
+        p = nullptr;
+    }
+    if (!p) {
+        int cap = 256;
+        while (cap < count) cap *= 2;
+        int st = blasdsp::create_plan(cap, &p);
+        if (st != BLASDSP_SUCCESS) {
+            p = nullptr;
+            return st;
+        }
+    }
+    return blasdsp::run_plan(p, descs, count, s);
+}
+
+}  // extern "C"

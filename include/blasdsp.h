@@ -180,6 +180,38 @@ BLASDSP_API int blasdsp_sgemm_split_bf16(char transA, char transB, int64_t M, in
                                          const float *B, int64_t ldb, float beta,
                                          float *C, int64_t ldc, int nterms, void *stream);
 
+/* ---- grouped execution (no counterpart in the reference) -----------------
+ * Many GEMMs of uneven shapes, transpose modes and precisions in one
+ * persistent launch per precision.  Each problem has the semantics of
+ * blasdsp_sgemm / blasdsp_dgemm (alpha, beta are cast to float for FP32
+ * problems) and produces the same bits as the tile kernel on its own. */
+#define BLASDSP_FLOAT32 0
+#define BLASDSP_FLOAT64 1
+typedef struct blasdsp_gemm_desc {
+    int dtype;              /* BLASDSP_FLOAT32 or BLASDSP_FLOAT64 */
+    char transA, transB;    /* 'N' or 'T' */
+    int64_t M, N, K;
+    double alpha, beta;
+    const void *A;
+    int64_t lda;
+    const void *B;
+    int64_t ldb;
+    void *C;
+    int64_t ldc;
+} blasdsp_gemm_desc;
+
+/* Runs the whole group asynchronously on `stream` (a cached plan per stream). */
+BLASDSP_API int blasdsp_gemm_grouped(const blasdsp_gemm_desc *descs, int count, void *stream);
+
+/* A reusable plan: device problem list, pinned staging and an instantiated
+ * hipGraph of the grouped kernels.  run() uploads the list and replays the
+ * graph on `stream`.  A plan is used by one host thread at a time. */
+typedef struct blasdsp_group_plan blasdsp_group_plan;
+BLASDSP_API int blasdsp_group_plan_create(int max_problems, blasdsp_group_plan **plan);
+BLASDSP_API int blasdsp_group_plan_run(blasdsp_group_plan *plan, const blasdsp_gemm_desc *descs,
+                                       int count, void *stream);
+BLASDSP_API int blasdsp_group_plan_destroy(blasdsp_group_plan *plan);
+
 /* ---- synthetic inputs ----------------------------------------------------
  * Fill the logical rows x cols matrix stored at X (device memory, leading
  * dimension ld, transposed storage when trans != 0) from the counter-based
