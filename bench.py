@@ -11,6 +11,19 @@ and the CPU baseline: the restated reference host GEMM (oracle, OpenMP) timed
 on this host over a bounded sample of rows of the same problem.
 
     python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--no-cpu]
+
+Other workloads (one JSON line each, same keys):
+    --workload skinny     shortA / thinB, FP32 and FP64, four modes, small side
+                          8/16/32, K = 8192, long side 65536, N(0,1) seed 2;
+                          HBM roofline
+    --workload precision  SGEMM NN 8192^3, U[-1,1) and log-uniform 2^[-8,8)
+                          seed 3: plain FP32 vs split-BF16 (3 and 6 terms),
+                          normwise error vs an FP64 product of the same inputs
+    --workload serving    10,000 requests (M, N, K log-uniform in [16, 4096],
+                          30 % shortA/thinB with small side 8-32, modes
+                          uniform, 70 % FP32 / 30 % FP64, seed 4), 64 GB
+                          budget: grouped continuous batching vs one launch
+                          per request, saturation and Poisson arrivals
 """
 from __future__ import annotations
 
@@ -89,10 +102,14 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--workload", default="dgemm", choices=["dgemm", "skinny", "precision", "serving"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1"}))
         sys.exit(2)
+    if args.workload != "dgemm":
+        {"skinny": bench_skinny, "precision": bench_precision, "serving": bench_serving}[args.workload](args)
+        return
 
     import torch
     import __graft_entry__
@@ -174,6 +191,196 @@ def main():
         },
         "cpu_baseline": None if args.no_cpu else cpu_baseline(),
     }
+    print(json.dumps(out), flush=True)
+
+
+def _common(args, metric_value, unit, dtype, config, roofline, cpu=None, ms_per_step=None):
+    return {
+        "metric": "TFLOP/s (% of roofline) per shape/precision; served GEMMs/s + p50/p99 latency",
+        "value": metric_value, "unit": unit, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": dtype, "data": "synthetic (counter-based generator on device)", "config": config,
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+
+
+def _time_events(fn, reps, stream):
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for e0, e1 in ev:
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+    torch.cuda.synchronize()
+    return statistics.mean(e0.elapsed_time(e1) for e0, e1 in ev)
+
+
+def bench_skinny(args):
+    """shortA (M small) and thinB (N small) through the irr_* entry points."""
+    import torch
+    import __graft_entry__
+
+    blas = __graft_entry__.load_package()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    K, LONG = 8192, 65536
+    reps = max(3, min(args.steps, 10))
+    cases = []
+    tot_bytes = tot_flops = tot_ms = 0.0
+    for dt, s in ((torch.float32, 4), (torch.float64, 8)):
+        for kind in ("shortA", "thinB"):
+            # long operand and its transposed copy: shortA streams B (K x LONG),
+            # thinB streams A (LONG x K)
+            big = torch.empty((K, LONG) if kind == "shortA" else (LONG, K), dtype=dt, device=dev)
+            big_t = torch.empty((LONG, K) if kind == "shortA" else (K, LONG), dtype=dt, device=dev)
+            blas.fill(big, big.shape[0], big.shape[1], blas.NORMAL, 2)
+            blas.fill(big_t, big.shape[0], big.shape[1], blas.NORMAL, 2, trans=True)
+            for small in (8, 16, 32):
+                sm = torch.empty((small, K) if kind == "shortA" else (K, small), dtype=dt, device=dev)
+                sm_t = torch.empty((K, small) if kind == "shortA" else (small, K), dtype=dt, device=dev)
+                blas.fill(sm, sm.shape[0], sm.shape[1], blas.NORMAL, 3)
+                blas.fill(sm_t, sm.shape[0], sm.shape[1], blas.NORMAL, 3, trans=True)
+                M, N = (small, LONG) if kind == "shortA" else (LONG, small)
+                C = torch.empty((M, N), dtype=dt, device=dev)
+                blas.fill(C, M, N, blas.NORMAL, 4)
+                for mode in MODES:
+                    if kind == "shortA":
+                        a = sm if mode[0] == "N" else sm_t
+                        b = big if mode[1] == "N" else big_t
+                    else:
+                        a = big if mode[0] == "N" else big_t
+                        b = sm if mode[1] == "N" else sm_t
+                    ms = _time_events(lambda: blas.gemm(mode[0], mode[1], a, b, C, 1.0, 1.0,
+                                                        algo=blas.ALGO_SKINNY), reps, stream)
+                    flops = 2.0 * M * N * K
+                    nbytes = s * (M * K + K * N + 2 * M * N)
+                    cases.append({"dtype": "f32" if s == 4 else "f64", "kind": kind, "small": small, "mode": mode,
+                                  "ms": round(ms, 4), "tflops": round(flops / ms / 1e9, 2),
+                                  "gbs": round(nbytes / ms / 1e6, 1),
+                                  "hbm_frac": round(nbytes / ms / 1e6 / PEAK_HBM_GBS, 4)})
+                    tot_bytes += nbytes
+                    tot_flops += flops
+                    tot_ms += ms
+                del sm, sm_t, C
+            del big, big_t
+            torch.cuda.empty_cache()
+    gbs = tot_bytes / tot_ms / 1e6
+    out = _common(args, round(tot_flops / tot_ms / 1e9, 3), "TFLOP/s", "f32+f64",
+                  {"workload": "shortA/thinB, small side 8/16/32, K=8192, long side 65536, NN/NT/TN/TT, "
+                               "FP32 and FP64, alpha=beta=1, N(0,1)", "cases": cases},
+                  {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                   "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None},
+                  ms_per_step=round(tot_ms / len(cases), 4))
+    print(json.dumps(out), flush=True)
+
+
+def bench_precision(args):
+    """SGEMM NN 8192^3: plain FP32 MFMA vs split-BF16, error vs FP64."""
+    import torch
+    import __graft_entry__
+
+    blas = __graft_entry__.load_package()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    n = 8192
+    reps = max(2, min(args.steps, 5))
+    flops = 2.0 * n ** 3
+    results = {}
+    for dist_name, dist in (("uniform", blas.UNIFORM), ("loguniform", blas.LOGUNIF)):
+        A = torch.empty((n, n), dtype=torch.float32, device=dev)
+        B = torch.empty((n, n), dtype=torch.float32, device=dev)
+        blas.fill(A, n, n, dist, 3)
+        blas.fill(B, n, n, dist, 30003)
+        # FP64 product of the same FP32 inputs (the FP64 DGEMM is itself
+        # checked against the extended-precision oracle in tests/)
+        ref = torch.zeros((n, n), dtype=torch.float64, device=dev)
+        blas.gemm("N", "N", A.double(), B.double(), ref, 1.0, 0.0)
+        ref_norm = torch.linalg.norm(ref).item()
+        C = torch.empty((n, n), dtype=torch.float32, device=dev)
+        row = {}
+        variants = (("fp32_mfma", lambda: blas.gemm("N", "N", A, B, C, 1.0, 0.0)),
+                    ("split_bf16_6", lambda: blas.sgemm_split_bf16("N", "N", A, B, C, 1.0, 0.0, nterms=6)),
+                    ("split_bf16_3", lambda: blas.sgemm_split_bf16("N", "N", A, B, C, 1.0, 0.0, nterms=3)))
+        for name, fn in variants:
+            ms = _time_events(fn, reps, stream)
+            err = (torch.linalg.norm(C.double() - ref).item() / ref_norm)
+            row[name] = {"ms": round(ms, 3), "tflops": round(flops / ms / 1e9, 2), "normwise_err": err}
+        results[dist_name] = row
+        del A, B, C, ref
+        torch.cuda.empty_cache()
+    u = results["uniform"]
+    speedup = u["fp32_mfma"]["ms"] / u["split_bf16_6"]["ms"]
+    out = _common(args, u["split_bf16_6"]["tflops"], "TFLOP/s", "bf16 x3 (fp32 accumulate)",
+                  {"workload": "SGEMM NN 8192^3 precision study, U[-1,1) and |x| = 2^U[-8,8), seed 3",
+                   "results": results, "split_bf16_6_over_fp32": round(speedup, 3)},
+                  {"bound": "mfma", "achieved": u["split_bf16_6"]["tflops"] * 6, "peak": 2500.0,
+                   "unit": "TFLOP/s", "frac": round(u["split_bf16_6"]["tflops"] * 6 / 2500.0, 4), "traffic": None})
+    print(json.dumps(out), flush=True)
+
+
+def serving_requests(blas, n=10000, seed=4, rate=None):
+    """The serving mix; arrival times Poisson at `rate` per second (None: 0)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    lo, hi = np.log(16), np.log(4096)
+    reqs = []
+    t = 0.0
+    for i in range(n):
+        M, N, K = (int(round(x)) for x in np.exp(rng.uniform(lo, hi, 3)))
+        if rng.random() < 0.3:
+            side = int(rng.integers(8, 33))
+            if rng.random() < 0.5:
+                M = side
+            else:
+                N = side
+        mode = MODES[rng.integers(4)]
+        dt = blas.FLOAT32 if rng.random() < 0.7 else blas.FLOAT64
+        if rate:
+            t += rng.exponential(1.0 / rate)
+        reqs.append(blas.Request(dt, mode[0].encode(), mode[1].encode(), M, N, K, 1.0, 1.0, 4_000_000 + i, t))
+    return reqs
+
+
+def bench_serving(args):
+    # The full 10,000-request run aborted once on the GPU with host heap
+    # corruption ("free(): corrupted unsorted chunks").  The same request mix
+    # through this code runs clean under AddressSanitizer on the CPU against a
+    # stand-in HIP runtime, and no cause has been shown, so the run is off
+    # until one is.
+    raise SystemExit("bench.py --workload serving is disabled: its full-size run aborted with host "
+                     "heap corruption and the cause has not been found")
+    import __graft_entry__
+
+    blas = __graft_entry__.load_package()
+    budget = 64 << 30
+    out_runs = {}
+
+    def run(grouped, rate, n=10000, batch=512, inflight=2):
+        reqs = serving_requests(blas, n=n, rate=rate)
+        cfg = blas.ServeConfig(budget, batch, inflight, grouped, 1.0 if rate else 0.0)
+        _, st = blas.serve(reqs, cfg)
+        d = st.as_dict()
+        return {k: (round(v, 6) if isinstance(v, float) else v) for k, v in d.items()}
+
+    sat_g = run(1, None)
+    sat_p = run(0, None)
+    out_runs["saturation_grouped"] = sat_g
+    out_runs["saturation_per_request"] = sat_p
+    # offered load at 80 % of the per-request loop's capacity: both keep up,
+    # latency compared at the same offered rate
+    rate = 0.8 * sat_p["gemms_per_s"]
+    out_runs["poisson_grouped"] = run(1, rate)
+    out_runs["poisson_per_request"] = run(0, rate)
+    out = _common(args, sat_g["gemms_per_s"], "GEMMs/s", "f32+f64",
+                  {"workload": "10,000 GEMM requests, M/N/K log-uniform [16,4096], 30% shortA/thinB (8-32), "
+                               "modes uniform, 70% FP32 / 30% FP64, seed 4, 64 GB budget",
+                   "offered_rate_per_s": round(rate, 1), "runs": out_runs,
+                   "grouped_over_per_request_throughput": round(sat_g["gemms_per_s"] / sat_p["gemms_per_s"], 3)},
+                  None)
     print(json.dumps(out), flush=True)
 
 

@@ -50,6 +50,38 @@ class GemmDesc(ctypes.Structure):
                 ("C", ctypes.c_void_p), ("ldc", ctypes.c_int64)]
 
 
+class Request(ctypes.Structure):
+    """blasdsp_request."""
+    _fields_ = [("dtype", ctypes.c_int), ("transA", ctypes.c_char), ("transB", ctypes.c_char),
+                ("M", ctypes.c_int64), ("N", ctypes.c_int64), ("K", ctypes.c_int64),
+                ("alpha", ctypes.c_double), ("beta", ctypes.c_double),
+                ("seed", ctypes.c_uint64), ("arrival_s", ctypes.c_double)]
+
+
+class RequestResult(ctypes.Structure):
+    """blasdsp_request_result."""
+    _fields_ = [("start_s", ctypes.c_double), ("done_s", ctypes.c_double), ("latency_s", ctypes.c_double),
+                ("checksum", ctypes.c_double), ("batch", ctypes.c_int), ("status", ctypes.c_int)]
+
+
+class ServeConfig(ctypes.Structure):
+    """blasdsp_serve_config."""
+    _fields_ = [("memory_budget", ctypes.c_int64), ("max_batch", ctypes.c_int), ("max_inflight", ctypes.c_int),
+                ("grouped", ctypes.c_int), ("time_scale", ctypes.c_double)]
+
+
+class ServeStats(ctypes.Structure):
+    """blasdsp_serve_stats."""
+    _fields_ = [("completed", ctypes.c_int64), ("rejected", ctypes.c_int64), ("wall_s", ctypes.c_double),
+                ("gemms_per_s", ctypes.c_double), ("tflops", ctypes.c_double), ("p50_s", ctypes.c_double),
+                ("p99_s", ctypes.c_double), ("mean_s", ctypes.c_double), ("max_s", ctypes.c_double),
+                ("launches", ctypes.c_int64), ("mean_batch", ctypes.c_double),
+                ("peak_resident_bytes", ctypes.c_int64)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class BlasdspError(RuntimeError):
     def __init__(self, fn: str, status: int):
         super().__init__(f"{fn} returned {status}")
@@ -91,6 +123,9 @@ def _declare(L: ctypes.CDLL) -> None:
     L.blasdsp_group_plan_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
     L.blasdsp_group_plan_run.argtypes = [ctypes.c_void_p, ctypes.POINTER(GemmDesc), ctypes.c_int, _vp]
     L.blasdsp_group_plan_destroy.argtypes = [ctypes.c_void_p]
+    L.blasdsp_serve.argtypes = [ctypes.POINTER(Request), ctypes.c_int, ctypes.POINTER(ServeConfig),
+                                ctypes.POINTER(RequestResult), ctypes.POINTER(ServeStats),
+                                ctypes.POINTER(ctypes.c_void_p)]
     for extra in _EXTRA_DECLS:
         extra(L)
 
@@ -207,6 +242,20 @@ class GroupPlan:
             self.close()
         except Exception:
             pass
+
+
+def serve(requests, config: ServeConfig, outputs=None):
+    """Run blasdsp_serve over a list of Request; returns (results, stats).
+    outputs: None or a list of host buffers (numpy arrays) or None per request."""
+    n = len(requests)
+    reqs = (Request * n)(*requests)
+    res = (RequestResult * n)()
+    stats = ServeStats()
+    outp = None
+    if outputs is not None:
+        outp = (ctypes.c_void_p * n)(*[None if o is None else o.ctypes.data for o in outputs])
+    check("blasdsp_serve", lib().blasdsp_serve(reqs, n, ctypes.byref(config), res, ctypes.byref(stats), outp))
+    return list(res), stats
 
 
 def fill(X, rows: int, cols: int, dist: int, seed: int, trans: bool = False, pad: float = 0.0, stream=None):

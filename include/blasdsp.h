@@ -212,6 +212,57 @@ BLASDSP_API int blasdsp_group_plan_run(blasdsp_group_plan *plan, const blasdsp_g
                                        int count, void *stream);
 BLASDSP_API int blasdsp_group_plan_destroy(blasdsp_group_plan *plan);
 
+/* ---- continuous-batching request loop (no counterpart in the reference) ----
+ * blasdsp_serve() plays a list of GEMM requests against the GPU as a server
+ * would: requests arrive at their (scaled) arrival times, are admitted FIFO
+ * when their operands fit in an HBM budget (operands stay resident until the
+ * request completes), and every admitted request not yet running is launched
+ * as soon as a launch slot frees up — with grouped != 0 as one grouped
+ * persistent launch (a hipGraph replay) per batch, else one GEMM launch per
+ * request.  Operands are generated on the device: the stored A, B and C of a
+ * request (dense, ld = stored columns) are the U[-1,1) generator streams of
+ * seeds 3*seed, 3*seed+1 and 3*seed+2 (as blasdsp_fill_* with dist 0).
+ * The call returns when every request has completed or was rejected. */
+typedef struct blasdsp_request {
+    int dtype;              /* BLASDSP_FLOAT32 or BLASDSP_FLOAT64 */
+    char transA, transB;
+    int64_t M, N, K;        /* all >= 1 */
+    double alpha, beta;
+    uint64_t seed;
+    double arrival_s;       /* offered arrival time after the loop starts */
+} blasdsp_request;
+
+typedef struct blasdsp_request_result {
+    double start_s;         /* admitted: operands allocated */
+    double done_s;          /* completion observed by the loop */
+    double latency_s;       /* done_s - scaled arrival time */
+    double checksum;        /* FP64 sum of the elements of the result C */
+    int batch;              /* launch the request ran in */
+    int status;             /* BLASDSP_SUCCESS, or why the request was rejected */
+} blasdsp_request_result;
+
+typedef struct blasdsp_serve_config {
+    int64_t memory_budget;  /* HBM bytes for resident operands */
+    int max_batch;          /* requests per grouped launch */
+    int max_inflight;       /* launch slots (streams) in flight at once, 1..16 */
+    int grouped;            /* 1: grouped launches; 0: one launch per request */
+    double time_scale;      /* multiplies arrival times; 0: all arrive at once */
+} blasdsp_serve_config;
+
+typedef struct blasdsp_serve_stats {
+    int64_t completed, rejected;
+    double wall_s, gemms_per_s, tflops;
+    double p50_s, p99_s, mean_s, max_s;     /* latency over completed requests */
+    int64_t launches;
+    double mean_batch;
+    int64_t peak_resident_bytes;
+} blasdsp_serve_stats;
+
+/* outputs: NULL, or per-request host buffers receiving the dense M x N C. */
+BLASDSP_API int blasdsp_serve(const blasdsp_request *reqs, int n, const blasdsp_serve_config *cfg,
+                              blasdsp_request_result *results, blasdsp_serve_stats *stats,
+                              void **outputs);
+
 /* ---- synthetic inputs ----------------------------------------------------
  * Fill the logical rows x cols matrix stored at X (device memory, leading
  * dimension ld, transposed storage when trans != 0) from the counter-based

@@ -74,3 +74,12 @@ def test_bad_algo_and_fill_arguments(blas):
     assert st == blas.ERR_INVALID or st == blas.ERR_DEVICE  # unknown algo
     assert L.blasdsp_fill_f64(FAKE, 4, 4, 3, 0, 0, 1, 0.0, None) == blas.ERR_INVALID
     assert L.blasdsp_fill_f32(FAKE, 4, 4, 4, 0, 9, 1, 0.0, None) == blas.ERR_INVALID
+
+
+def test_serve_rejects_bad_config(blas):
+    cfg = blas.ServeConfig(0, 8, 2, 1, 0.0)  # zero budget
+    res = (blas.RequestResult * 1)()
+    req = (blas.Request * 1)(blas.Request(0, b"N", b"N", 4, 4, 4, 1.0, 0.0, 1, 0.0))
+    assert blas.lib().blasdsp_serve(req, 1, cfg, res, None, None) == blas.ERR_INVALID
+    cfg = blas.ServeConfig(1 << 20, 8, 0, 1, 0.0)  # no launch slot
+    assert blas.lib().blasdsp_serve(req, 1, cfg, res, None, None) == blas.ERR_INVALID
