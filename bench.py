@@ -24,6 +24,11 @@ Other workloads (one JSON line each, same keys):
                           uniform, 70 % FP32 / 30 % FP64, seed 4), 64 GB
                           budget: grouped continuous batching vs one launch
                           per request, saturation and Poisson arrivals
+    --workload grouped    500 FP32 GEMMs with M, N, K uniform in [16, 256] (seed 0),
+                          NN: one grouped persistent launch vs one launch each
+    --workload copy       a plain 2 GiB device-to-device copy (torch): the
+                          achievable HBM bandwidth, and the reference kernel
+                          that scales rocprofv3 FETCH_SIZE / WRITE_SIZE
 """
 from __future__ import annotations
 
@@ -49,7 +54,7 @@ MODES = ("NN", "NT", "TN", "TT")
 def traffic_from_profiles():
     """HBM bytes per dgemm launch measured with rocprofv3 --pmc (FETCH_SIZE,
     WRITE_SIZE), if the summary has been recorded under profiles/."""
-    p = ROOT / "profiles" / "dgemm_4096_nn_hbm_traffic.json"
+    p = ROOT / "profiles" / "dgemm_4096_hbm_traffic.json"
     if not p.exists():
         return None
     try:
@@ -102,13 +107,14 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--workload", default="dgemm", choices=["dgemm", "skinny", "precision", "serving"])
+    ap.add_argument("--workload", default="dgemm", choices=["dgemm", "skinny", "precision", "serving", "copy", "grouped"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1"}))
         sys.exit(2)
     if args.workload != "dgemm":
-        {"skinny": bench_skinny, "precision": bench_precision, "serving": bench_serving}[args.workload](args)
+        {"skinny": bench_skinny, "precision": bench_precision, "serving": bench_serving,
+         "copy": bench_copy, "grouped": bench_grouped}[args.workload](args)
         return
 
     import torch
@@ -318,6 +324,53 @@ def bench_precision(args):
                    "results": results, "split_bf16_6_over_fp32": round(speedup, 3)},
                   {"bound": "mfma", "achieved": u["split_bf16_6"]["tflops"] * 6, "peak": 2500.0,
                    "unit": "TFLOP/s", "frac": round(u["split_bf16_6"]["tflops"] * 6 / 2500.0, 4), "traffic": None})
+    print(json.dumps(out), flush=True)
+
+
+def bench_copy(args):
+    import torch
+
+    n = 1 << 29  # 2 GiB of FP32
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    stream = torch.cuda.current_stream()
+    ms = _time_events(lambda: dst.copy_(src), max(3, min(args.steps, 20)), stream)
+    gbs = 2 * 4 * n / ms / 1e6
+    out = _common(args, round(gbs, 1), "GB/s", "f32",
+                  {"workload": "device copy of 2 GiB (read 2 GiB + write 2 GiB per launch)",
+                   "bytes_read": 4 * n, "bytes_written": 4 * n, "ms": round(ms, 4)},
+                  {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                   "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None})
+    print(json.dumps(out), flush=True)
+
+
+def bench_grouped(args):
+    import numpy as np
+    import torch
+    import __graft_entry__
+
+    blas = __graft_entry__.load_package()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    rng = np.random.default_rng(0)
+    probs = []
+    for _ in range(500):
+        M, N, K = (int(x) for x in rng.integers(16, 257, 3))
+        A = torch.empty((M, K), device=dev)
+        B = torch.empty((K, N), device=dev)
+        blas.fill(A, M, K, blas.UNIFORM, 1)
+        blas.fill(B, K, N, blas.UNIFORM, 2)
+        probs.append((A, B, torch.zeros((M, N), device=dev)))
+    descs = [blas.desc("N", "N", a, b, c) for a, b, c in probs]
+    flops = sum(2.0 * a.shape[0] * b.shape[1] * a.shape[1] for a, b, _ in probs)
+    reps = max(3, min(args.steps, 20))
+    ms_each = _time_events(lambda: [blas.gemm("N", "N", a, b, c) for a, b, c in probs], reps, stream)
+    ms_group = _time_events(lambda: blas.gemm_grouped(descs), reps, stream)
+    out = _common(args, round(flops / ms_group / 1e9, 3), "TFLOP/s", "f32",
+                  {"workload": "500 FP32 GEMMs, M/N/K uniform in [16, 256], NN",
+                   "ms_grouped": round(ms_group, 4), "ms_one_launch_each": round(ms_each, 4),
+                   "speedup": round(ms_each / ms_group, 2)},
+                  None, ms_per_step=round(ms_group, 4))
     print(json.dumps(out), flush=True)
 
 

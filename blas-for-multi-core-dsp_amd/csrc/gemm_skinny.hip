@@ -16,19 +16,21 @@
 // index maps instead of executed.
 //
 // Each wave owns a 64-column panel of O (all Ms rows, MT = Ms/16 MFMA row
-// tiles x 4 column tiles) and walks K in blocks of 16.  L is read straight
+// tiles x 4 column tiles) and walks K in blocks of one 128-byte line (32
+// FP32 or 16 FP64 values).  L is read straight
 // from HBM into MFMA operand registers with 16-byte vector loads and no LDS
 // round trip: every element of L is loaded exactly once.  The MFMA K index is
 // permuted so that each lane's vector holds whole operands:
 //   * L stored k-major ([k][nl], nl contiguous): lane (c, g) = (l%16, l/16)
-//     loads row k0+4g+j, columns n0 + V*c .. + V-1 at virtual step j; element
-//     e of the vector is the B operand of column tile e, which therefore
-//     covers columns n0 + V*c + e (an interleaved column map undone at the
-//     store);
-//   * L stored nl-major ([nl][k], k contiguous): lane (c, g) loads the 4-run
-//     k0+4g .. k0+4g+3 of row n0+16t+c, one run per column tile t.
-// In both cases virtual step j pairs lane group g with k = k0 + 4g + j, and S
-// is read with the same pairing (a 4-run along k when S is k-contiguous).
+//     loads row k0 + g*R + j, columns n0 + V*c .. + V-1 at virtual step j
+//     (R = block / 4); element e of the vector is the B operand of column
+//     tile e, which therefore covers columns n0 + V*c + e (an interleaved
+//     column map undone at the store);
+//   * L stored nl-major ([nl][k], k contiguous): lane (c, g) loads the run
+//     k0 + g*R .. + R-1 of row n0+16t+c, one run per column tile t, so the
+//     four lane groups read one whole 128-byte line of each row.
+// In both cases virtual step j pairs lane group g with k = k0 + g*R + j, and
+// S is read with the same pairing (a run along k when S is k-contiguous).
 // S is tiny (<= 64 x K) and is served from L1/L2.
 //
 // When the Nl panels alone do not give >= 4 workgroups per CU, K is split
@@ -79,7 +81,7 @@ struct SkinnyArgs {
     int64_t ldc;
     T *W;             // split partials [split][Ms][Nl], used when splitk > 1
     int64_t Ms, Nl, K;
-    int64_t kslice;   // K per slice, a multiple of 16
+    int64_t kslice;   // K per slice, a multiple of the K block
     int splitk;
     T alpha, beta;
     bool l_vec, s_vec;
@@ -87,7 +89,17 @@ struct SkinnyArgs {
 
 constexpr int kWaves = 4;      // waves per workgroup, each an independent panel
 constexpr int kPanel = 64;     // O columns per wave
-constexpr int kBlockK = 16;    // K per step
+
+// K per block: one 128-byte line of a k-contiguous row (32 FP32 / 16 FP64).
+template <typename T>
+struct Blk {
+    static constexpr int V = SK<T>::V;            // elements per 16-byte vector
+    static constexpr int KB = 128 / (int)sizeof(T);
+    static constexpr int RUN = KB / 4;            // k-run per lane group (8 / 4)
+    static constexpr int NSTEP = RUN;             // MFMA k-steps per block
+    static constexpr int RV = RUN / V;            // vectors per run (2)
+    static constexpr int NLV = 4 * RV;            // L vectors per lane per block (8)
+};
 
 // Column of O held by lane column c of MFMA column tile t.
 template <typename T, bool L_KC>
@@ -99,27 +111,23 @@ __device__ __forceinline__ int col_of(int t, int c)
     return (t / V) * 16 * V + V * c + (t % V);
 }
 
-// Registers for one 16-deep K block of one wave.
+// Registers for one K block of one wave.  Virtual MFMA step j pairs lane
+// group g with k = k0 + g * RUN + j on both operands.
 template <typename T, int MT, bool S_KC, bool L_KC>
 struct Block {
     using vec_t = typename SK<T>::vec_t;
-    static constexpr int V = SK<T>::V;
-    static constexpr int NLV = 16 / V;   // 16-byte vectors of L per lane per block (4 or 8)
-    static constexpr int NSV = S_KC ? 4 / V : 0;
-    vec_t l[NLV];
-    // S: k-contiguous -> 4-runs as vectors; m-contiguous -> scalars per step
-    vec_t s_run[MT][S_KC ? (4 / V) : 1];
-    T s_scl[MT][S_KC ? 1 : 4];
+    using B_ = Blk<T>;
+    static constexpr int V = B_::V;
+    vec_t l[B_::NLV];
+    // S: k-contiguous -> runs as vectors; m-contiguous -> scalars per step
+    vec_t s_run[MT][S_KC ? B_::RV : 1];
+    T s_scl[MT][S_KC ? 1 : B_::NSTEP];
 
     // L operand of column tile t at virtual step j.
     __device__ __forceinline__ T lop(int t, int j) const
     {
-        if (L_KC) {
-            const int idx = t * (4 / V) + j / V;   // t's 4-run, vector j/V
-            return l[idx][j % V];
-        }
-        // nl-contiguous: vector (j, h) with h = t / V, element e = t % V
-        return l[j * (4 / V) + t / V][t % V];
+        if (L_KC) return l[t * B_::RV + j / V][j % V];  // t's run, vector j / V
+        return l[j * (4 / V) + t / V][t % V];           // step j's row, half t / V
     }
     __device__ __forceinline__ T sop(int mt, int j) const
     {
@@ -133,16 +141,18 @@ __device__ __forceinline__ void load_block(Block<T, MT, S_KC, L_KC> &b, const Sk
                                            int64_t n0, int64_t k0, int64_t kend, int c, int g)
 {
     using vec_t = typename SK<T>::vec_t;
-    constexpr int V = SK<T>::V;
+    using B_ = Blk<T>;
+    constexpr int V = B_::V, RUN = B_::RUN;
     // ---- L ----
     if (L_KC) {
-        // t-th column tile: row n0 + 16t + c, k-run k0 + 4g .. +3 in 4/V vectors
+        // column tile t: row n0 + 16t + c, k-run k0 + g*RUN .. + RUN-1 (a full
+        // 128-byte line per row across the four lane groups)
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
 #pragma unroll
-            for (int q = 0; q < 4 / V; ++q) {
+            for (int q = 0; q < B_::RV; ++q) {
                 const int64_t row = n0 + 16 * t + c;
-                const int64_t kk = k0 + 4 * g + q * V;
+                const int64_t kk = k0 + g * RUN + q * V;
                 vec_t v;
                 if (!MASK) {
                     v = *reinterpret_cast<const vec_t *>(a.L + row * a.ldl + kk);
@@ -153,16 +163,16 @@ __device__ __forceinline__ void load_block(Block<T, MT, S_KC, L_KC> &b, const Sk
                     for (int e = 0; e < V; ++e)
                         v[e] = (row < a.Nl && kk + e < kend) ? a.L[row * a.ldl + kk + e] : T(0);
                 }
-                b.l[t * (4 / V) + q] = v;
+                b.l[t * B_::RV + q] = v;
             }
         }
     } else {
-        // step j: row k0 + 4g + j, columns n0 + h*16V + V*c .. + V-1
+        // step j: row k0 + g*RUN + j, columns n0 + h*16V + V*c .. + V-1
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < B_::NSTEP; ++j) {
 #pragma unroll
             for (int h = 0; h < 4 / V; ++h) {
-                const int64_t kk = k0 + 4 * g + j;
+                const int64_t kk = k0 + g * RUN + j;
                 const int64_t col = n0 + h * 16 * V + V * c;
                 vec_t v;
                 if (!MASK) {
@@ -184,8 +194,8 @@ __device__ __forceinline__ void load_block(Block<T, MT, S_KC, L_KC> &b, const Sk
         const int64_t ms = mt * 16 + c;
         if (S_KC) {
 #pragma unroll
-            for (int q = 0; q < 4 / V; ++q) {
-                const int64_t kk = k0 + 4 * g + q * V;
+            for (int q = 0; q < B_::RV; ++q) {
+                const int64_t kk = k0 + g * RUN + q * V;
                 vec_t v;
                 if (ms < a.Ms && a.s_vec && (!MASK || kk + V <= kend)) {
                     v = *reinterpret_cast<const vec_t *>(a.S + ms * a.lds + kk);
@@ -198,8 +208,8 @@ __device__ __forceinline__ void load_block(Block<T, MT, S_KC, L_KC> &b, const Sk
             }
         } else {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int64_t kk = k0 + 4 * g + j;
+            for (int j = 0; j < B_::NSTEP; ++j) {
+                const int64_t kk = k0 + g * RUN + j;
                 b.s_scl[mt][j] = (ms < a.Ms && (!MASK || kk < kend)) ? a.S[kk * a.lds + ms] : T(0);
             }
         }
@@ -211,7 +221,7 @@ __device__ __forceinline__ void mma_block(typename SK<T>::acc_t (&acc)[MT][4],
                                           const Block<T, MT, S_KC, L_KC> &b)
 {
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < Blk<T>::NSTEP; ++j)
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
@@ -236,16 +246,17 @@ __global__ void __launch_bounds__(256) skinny_kernel(SkinnyArgs<T> a)
 #pragma unroll
         for (int t = 0; t < 4; ++t) acc[i][t] = acc_t{0, 0, 0, 0};
 
-    const bool panel_full = a.l_vec && (L_KC ? (n0 + kPanel <= a.Nl) : (n0 + kPanel <= a.Nl));
-    const int64_t kfull = panel_full ? kb + ((ke - kb) / kBlockK) * kBlockK : kb;
+    constexpr int KB = Blk<T>::KB;
+    const bool panel_full = a.l_vec && (n0 + kPanel <= a.Nl);
+    const int64_t kfull = panel_full ? kb + ((ke - kb) / KB) * KB : kb;
 
     // Full blocks: unmasked L loads, register double buffer.
     int64_t k0 = kb;
     if (k0 < kfull) {
         Block<T, MT, S_KC, L_KC> cur, nxt;
         load_block<T, MT, S_KC, L_KC, false>(cur, a, n0, k0, ke, c, g);
-        for (; k0 + kBlockK < kfull; k0 += kBlockK) {
-            load_block<T, MT, S_KC, L_KC, false>(nxt, a, n0, k0 + kBlockK, ke, c, g);
+        for (; k0 + KB < kfull; k0 += KB) {
+            load_block<T, MT, S_KC, L_KC, false>(nxt, a, n0, k0 + KB, ke, c, g);
             mma_block<T, MT, S_KC, L_KC>(acc, cur);
             cur = nxt;
         }
@@ -253,7 +264,7 @@ __global__ void __launch_bounds__(256) skinny_kernel(SkinnyArgs<T> a)
         k0 = kfull;
     }
     // Ragged tail (partial panel or partial K block): masked loads.
-    for (; k0 < ke; k0 += kBlockK) {
+    for (; k0 < ke; k0 += KB) {
         Block<T, MT, S_KC, L_KC> b;
         load_block<T, MT, S_KC, L_KC, true>(b, a, n0, k0, ke, c, g);
         mma_block<T, MT, S_KC, L_KC>(acc, b);
@@ -379,9 +390,9 @@ hipError_t launch_skinny(const Gemm<T> &g, int splitk, T *work, hipStream_t s)
     roles(g, a, s_kc, l_kc, trans_out);
     if (a.Ms > 64 || a.Ms <= 0 || a.Nl <= 0) return hipErrorInvalidValue;
     if (splitk < 1) splitk = 1;
+    constexpr int KB = Blk<T>::KB;
     int64_t kslice = (g.K + splitk - 1) / splitk;
-    kslice = (kslice + kBlockK - 1) / kBlockK * kBlockK;
-    if (kslice < kBlockK) kslice = kBlockK;
+    kslice = (kslice + KB - 1) / KB * KB;
     const int64_t nsplit = (g.K + kslice - 1) / kslice;
     a.kslice = kslice;
     a.splitk = (int)(nsplit < 1 ? 1 : nsplit);
