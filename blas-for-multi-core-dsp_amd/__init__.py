@@ -82,6 +82,12 @@ class ServeStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class TuneChoice(ctypes.Structure):
+    """blasdsp_tune_choice."""
+    _fields_ = [("algo", ctypes.c_int), ("tile", ctypes.c_int), ("splitk", ctypes.c_int),
+                ("time_us", ctypes.c_double)]
+
+
 class BlasdspError(RuntimeError):
     def __init__(self, fn: str, status: int):
         super().__init__(f"{fn} returned {status}")
@@ -126,6 +132,12 @@ def _declare(L: ctypes.CDLL) -> None:
     L.blasdsp_serve.argtypes = [ctypes.POINTER(Request), ctypes.c_int, ctypes.POINTER(ServeConfig),
                                 ctypes.POINTER(RequestResult), ctypes.POINTER(ServeStats),
                                 ctypes.POINTER(ctypes.c_void_p)]
+    tune_args = [ctypes.c_int, ctypes.c_char, ctypes.c_char, _i64, _i64, _i64]
+    L.blasdsp_autotune.argtypes = tune_args + [_vp, ctypes.POINTER(TuneChoice)]
+    L.blasdsp_tune_query.argtypes = tune_args + [ctypes.POINTER(TuneChoice)]
+    L.blasdsp_tune_save.argtypes = [ctypes.c_char_p]
+    L.blasdsp_tune_load.argtypes = [ctypes.c_char_p]
+    L.blasdsp_tune_clear.argtypes = []
     for extra in _EXTRA_DECLS:
         extra(L)
 
@@ -256,6 +268,20 @@ def serve(requests, config: ServeConfig, outputs=None):
         outp = (ctypes.c_void_p * n)(*[None if o is None else o.ctypes.data for o in outputs])
     check("blasdsp_serve", lib().blasdsp_serve(reqs, n, ctypes.byref(config), res, ctypes.byref(stats), outp))
     return list(res), stats
+
+
+def autotune(dtype: int, transA: str, transB: str, M: int, N: int, K: int, stream=None) -> TuneChoice:
+    c = TuneChoice()
+    check("blasdsp_autotune", lib().blasdsp_autotune(dtype, transA.encode(), transB.encode(), M, N, K,
+                                                     _stream(stream), ctypes.byref(c)))
+    return c
+
+
+def tune_query(dtype: int, transA: str, transB: str, M: int, N: int, K: int):
+    """The recorded choice for the shape class, or None."""
+    c = TuneChoice()
+    st = lib().blasdsp_tune_query(dtype, transA.encode(), transB.encode(), M, N, K, ctypes.byref(c))
+    return c if st == SUCCESS else None
 
 
 def fill(X, rows: int, cols: int, dist: int, seed: int, trans: bool = False, pad: float = 0.0, stream=None):

@@ -18,6 +18,7 @@
 
 #include "../../include/blasdsp.h"
 #include "api_common.h"
+#include "tune.h"
 
 namespace blasdsp {
 
@@ -110,16 +111,26 @@ int run(const Gemm<T> &g, int algo, int splitk, hipStream_t s)
     if (g.M == 0 || g.N == 0) return BLASDSP_SUCCESS;
     if (g.K == 0 || g.alpha == T(0)) return hip_status(launch_scale(g, s));
     const int64_t small = g.M <= g.N ? g.M : g.N;
+    // 64 x 64 tiles when 128 x 128 tiles would leave most CUs idle
+    const int64_t t128 = ((g.M + 127) / 128) * ((g.N + 127) / 128);
+    int tile = t128 < num_cus() ? 64 : 128;
     bool use_skinny;
     if (algo == BLASDSP_ALGO_TILE) use_skinny = false;
     else if (algo == BLASDSP_ALGO_SKINNY) use_skinny = small <= 64;  // larger: tile kernel
-    else if (algo == BLASDSP_ALGO_AUTO) use_skinny = skinny_shape(g);
-    else return BLASDSP_ERR_INVALID;
-    if (!use_skinny) {
-        // 64 x 64 tiles when 128 x 128 tiles would leave most CUs idle
-        const int64_t t128 = ((g.M + 127) / 128) * ((g.N + 127) / 128);
-        return hip_status(launch_tile(g, t128 < num_cus() ? 64 : 128, s));
+    else if (algo == BLASDSP_ALGO_AUTO) {
+        TuneChoice c;
+        if (tune_lookup(sizeof(T) == 8, g.ta, g.tb, g.M, g.N, g.K, &c) &&
+            (c.algo == BLASDSP_ALGO_TILE || small <= 64)) {
+            use_skinny = c.algo == BLASDSP_ALGO_SKINNY;
+            tile = c.tile;
+            if (splitk <= 0) splitk = c.splitk;
+        } else {
+            use_skinny = skinny_shape(g);
+        }
+    } else {
+        return BLASDSP_ERR_INVALID;
     }
+    if (!use_skinny) return hip_status(launch_tile(g, tile, s));
 
     if (splitk <= 0) splitk = skinny_pick_splitk(g, num_cus());
     if (splitk > 256) return BLASDSP_ERR_INVALID;

@@ -19,6 +19,7 @@
 #include "api_common.h"
 #include "grouped.h"
 #include "tile_body.h"
+#include "tune.h"
 
 struct blasdsp_group_plan {
     int device = 0;
@@ -140,7 +141,11 @@ int to_problem(const blasdsp_gemm_desc &d, GroupProblem &p, bool &skip)
     p.ta = g.ta; p.tb = g.tb;
     p.a_vec = tile::vec_ok((const T *)d.A, d.lda);
     p.b_vec = tile::vec_ok((const T *)d.B, d.ldb);
-    p.bm = (d.M <= 96 || d.N <= 96) ? 64 : 128;
+    TuneChoice c;
+    if (tune_lookup(sizeof(T) == 8, g.ta, g.tb, d.M, d.N, d.K, &c) && c.algo == BLASDSP_ALGO_TILE)
+        p.bm = c.tile;
+    else
+        p.bm = (d.M <= 96 || d.N <= 96) ? 64 : 128;
     return BLASDSP_SUCCESS;
 }
 

@@ -263,6 +263,29 @@ BLASDSP_API int blasdsp_serve(const blasdsp_request *reqs, int n, const blasdsp_
                               blasdsp_request_result *results, blasdsp_serve_stats *stats,
                               void **outputs);
 
+/* ---- autotuner (no counterpart in the reference) --------------------------
+ * blasdsp_autotune() times the candidate kernels for one shape on `stream`
+ * (the tile kernel with 64 and 128 tiles; when min(M, N) <= 64 also the
+ * skinny kernel with split-K 1 .. 64) and records the fastest for the shape
+ * class (precision, modes, round(log2) of M, N and K).  The AUTO dispatch of
+ * every GEMM entry point and the grouped launch then use the recorded choice.
+ * It allocates and frees temporary operands (it synchronises the device).
+ * Tables persist as text through blasdsp_tune_save / _load. */
+typedef struct blasdsp_tune_choice {
+    int algo;               /* BLASDSP_ALGO_TILE or BLASDSP_ALGO_SKINNY */
+    int tile;               /* tile edge (64 or 128) for the tile kernel */
+    int splitk;             /* K slices for the skinny kernel */
+    double time_us;         /* measured time of one call */
+} blasdsp_tune_choice;
+BLASDSP_API int blasdsp_autotune(int dtype, char transA, char transB, int64_t M, int64_t N, int64_t K,
+                                 void *stream, blasdsp_tune_choice *out);
+/* BLASDSP_ERR_UNSUPPORTED when the shape class has no entry. */
+BLASDSP_API int blasdsp_tune_query(int dtype, char transA, char transB, int64_t M, int64_t N, int64_t K,
+                                   blasdsp_tune_choice *out);
+BLASDSP_API int blasdsp_tune_save(const char *path);
+BLASDSP_API int blasdsp_tune_load(const char *path);
+BLASDSP_API int blasdsp_tune_clear(void);
+
 /* ---- synthetic inputs ----------------------------------------------------
  * Fill the logical rows x cols matrix stored at X (device memory, leading
  * dimension ld, transposed storage when trans != 0) from the counter-based

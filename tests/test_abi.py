@@ -83,3 +83,27 @@ def test_serve_rejects_bad_config(blas):
     assert blas.lib().blasdsp_serve(req, 1, cfg, res, None, None) == blas.ERR_INVALID
     cfg = blas.ServeConfig(1 << 20, 8, 0, 1, 0.0)  # no launch slot
     assert blas.lib().blasdsp_serve(req, 1, cfg, res, None, None) == blas.ERR_INVALID
+
+
+def test_tune_table_load_query_save(blas, tmp_path):
+    L = blas.lib()
+    L.blasdsp_tune_clear()
+    assert blas.tune_query(blas.FLOAT64, "N", "N", 4096, 4096, 4096) is None
+    p = tmp_path / "tune.txt"
+    p.write_text("blasdsp-tune-v1 1 0 0 12 12 12 1 128 1 2400.0\n"
+                 "blasdsp-tune-v1 0 1 0 5 16 13 2 0 4 330.5\n")
+    assert L.blasdsp_tune_load(str(p).encode()) == 0
+    c = blas.tune_query(blas.FLOAT64, "N", "N", 4000, 4200, 4096)   # same log2 buckets
+    assert (c.algo, c.tile, c.splitk) == (blas.ALGO_TILE, 128, 1)
+    c = blas.tune_query(blas.FLOAT32, "T", "N", 32, 65536, 8192)
+    assert (c.algo, c.splitk) == (blas.ALGO_SKINNY, 4)
+    assert blas.tune_query(blas.FLOAT32, "N", "N", 32, 65536, 8192) is None
+    q = tmp_path / "out.txt"
+    assert L.blasdsp_tune_save(str(q).encode()) == 0
+    assert sorted(q.read_text().split("\n")) == sorted(p.read_text().replace("2400.0", "2400.000")
+                                                       .replace("330.5", "330.500").split("\n"))
+    bad = tmp_path / "bad.txt"
+    bad.write_text("blasdsp-tune-v1 1 0 0 12 12 12 1 96 1 1.0\n")  # tile 96 is not a kernel
+    assert L.blasdsp_tune_load(str(bad).encode()) == blas.ERR_INVALID
+    assert L.blasdsp_tune_load(str(tmp_path / "missing.txt").encode()) == blas.ERR_INVALID
+    L.blasdsp_tune_clear()
