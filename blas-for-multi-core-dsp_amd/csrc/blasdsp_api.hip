@@ -266,21 +266,44 @@ int sgemm_null(int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N, int6
     return blasdsp::null_entry<float>('N', 'N', lda, ldb, ldc, M, N, K, A, B, C, alpha, beta, stream);
 }
 
+}  // extern "C"
+
+namespace blasdsp {
+namespace {
+int split_entry(char transA, char transB, int64_t M, int64_t N, int64_t K, float alpha, const float *A,
+                int64_t lda, const float *B, int64_t ldb, float beta, float *C, int64_t ldc, int scheme,
+                void *stream)
+{
+    Gemm<float> g;
+    int st = check<float>(g, transA, transB, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+    if (st != BLASDSP_SUCCESS) return st;
+    if (M == 0 || N == 0) return BLASDSP_SUCCESS;
+    hipStream_t s = (hipStream_t)stream;
+    if (K == 0 || alpha == 0.0f) return hip_status(launch_scale(g, s));
+    void *work = nullptr;
+    hipError_t e = workspace(s, split_workspace_bytes(M, N, K, scheme), &work);
+    if (e != hipSuccess) return hip_status(e);
+    return hip_status(launch_split_gemm(g, scheme, work, s));
+}
+}  // namespace
+}  // namespace blasdsp
+
+extern "C" {
+
 int blasdsp_sgemm_split_bf16(char transA, char transB, int64_t M, int64_t N, int64_t K, float alpha,
                              const float *A, int64_t lda, const float *B, int64_t ldb, float beta,
                              float *C, int64_t ldc, int nterms, void *stream)
 {
-    blasdsp::Gemm<float> g;
-    int st = blasdsp::check<float>(g, transA, transB, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
-    if (st != BLASDSP_SUCCESS) return st;
     if (nterms != 3 && nterms != 6) return BLASDSP_ERR_INVALID;
-    if (M == 0 || N == 0) return BLASDSP_SUCCESS;
-    hipStream_t s = (hipStream_t)stream;
-    if (K == 0 || alpha == 0.0f) return blasdsp::hip_status(blasdsp::launch_scale(g, s));
-    void *work = nullptr;
-    hipError_t e = blasdsp::workspace(s, blasdsp::split_bf16_workspace_bytes(M, N, K, nterms), &work);
-    if (e != hipSuccess) return blasdsp::hip_status(e);
-    return blasdsp::hip_status(blasdsp::launch_split_bf16(g, nterms, work, s));
+    return blasdsp::split_entry(transA, transB, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                                nterms == 6 ? 0 : 1, stream);
+}
+
+int blasdsp_sgemm_split_fp16(char transA, char transB, int64_t M, int64_t N, int64_t K, float alpha,
+                             const float *A, int64_t lda, const float *B, int64_t ldb, float beta,
+                             float *C, int64_t ldc, void *stream)
+{
+    return blasdsp::split_entry(transA, transB, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, 2, stream);
 }
 
 int blasdsp_fill_f64(double *X, int64_t rows, int64_t cols, int64_t ld, int trans, int dist,

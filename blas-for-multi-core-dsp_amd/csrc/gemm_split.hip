@@ -1,28 +1,33 @@
-// gemm_split.hip — FP32 GEMM on the BF16 matrix cores ("split-BF16").
+// gemm_split.hip — FP32 GEMM on the 16-bit matrix cores ("split" forms).
 //
 // No counterpart in the reference (an added precision form).  Every FP32
-// operand x is split into BF16 pieces x = hi + mid + lo (+ a residual below
-// 2^-24 |x|), each piece the round-to-nearest-even BF16 of what the previous
-// pieces left over.  The product is assembled from the piece products whose
-// magnitude is at least 2^-16 of the full product:
-//   6 terms:  hh + hm + mh + hl + lh + mm   (three pieces; residual ~2^-24)
-//   3 terms:  hh + hl + lh with two pieces hi + lo (residual ~2^-17)
-// All products are exact in the FP32 MFMA accumulator inputs (8-bit x 8-bit
-// significands), so the error comes from the dropped terms, the split residual
-// and FP32 accumulation.  Accumulation is two-level: each 256-deep K chunk is
-// accumulated from zero in its own MFMA accumulators and then added into the
-// running sum, which keeps the rounding error of a K = 8192 sum near that of a
-// 32-term sum instead of an 8192-term one.
+// operand x is split into 16-bit pieces and the product is assembled from the
+// piece products that matter:
+//   BF16, 6 terms:  x = hi + mid + lo (each the round-to-nearest-even BF16 of
+//                   what is left, residual ~2^-24 |x|);
+//                   hh + hm + mh + hl + lh + mm (dropped terms ~2^-24)
+//   BF16, 3 terms:  x = hi + lo (residual ~2^-17 |x|); hh + hl + lh
+//   FP16, 3 terms:  rows of op(A) and columns of op(B) scaled by a power of two
+//                   (exact) so the largest magnitude is in [2^14, 2^15);
+//                   x' = hi + 2^-11 lo with 11-bit hi and lo (residual ~2^-22);
+//                   hh + 2^-11 (hl + lh), the cross terms in their own
+//                   accumulator, scales undone in the epilogue.
+// Piece products are exact in FP32 (8- or 11-bit significands), so the error
+// comes from the dropped terms, the split residual and FP32 accumulation.  The
+// BF16 forms accumulate in two levels: each 256-deep K chunk from zero in its
+// own MFMA accumulators, then added into the running sum.
 //
-// Two kernels:
-//   split_kernel   reads op(A) (or op(B)^T) in any storage order and writes the
-//                  pieces as BF16 planes [rows][Kp], K contiguous, zero padded
-//                  to whole 128-row tiles and 64-deep K blocks — the transpose
-//                  modes are absorbed here, through an LDS tile;
-//   planes_kernel  a 128 x 128-tile MFMA GEMM (v_mfma_f32_16x16x32_bf16) over
-//                  the planes: per 32-deep K step each wave reads every piece
-//                  fragment once from LDS and issues nterms x 16 MFMAs, so the
-//                  LDS traffic per MFMA falls as the number of terms grows.
+// Kernels:
+//   rowscale_kernel  (FP16 form) per-row power-of-two scales of op(A), op(B)^T;
+//   split_kernel     reads op(A) (or op(B)^T) in any storage order and writes
+//                    the pieces as planes [rows][Kp], K contiguous, zero padded
+//                    to whole 128-row tiles and 64-deep K blocks — the
+//                    transpose modes are absorbed here, through an LDS tile;
+//   planes_kernel    a 128 x 128-tile MFMA GEMM (v_mfma_f32_16x16x32_bf16 /
+//                    _f16) over the planes: per 32-deep K step each wave reads
+//                    every piece fragment once from LDS and issues
+//                    terms x 16 MFMAs.  The two-piece forms fit two
+//                    workgroups per CU (2 x 80 KiB of LDS).
 #include "internal.h"
 
 namespace blasdsp {
@@ -30,12 +35,13 @@ namespace {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kBM = 128, kBN = 128, kBK = 32;  // GEMM tile
 constexpr int kKpad = 64;                      // plane K padding (split tile)
 constexpr int kRowPad = 128;                   // plane row padding
-constexpr int kLdsStride = kBK + 8;            // bf16 per LDS row (80 bytes)
+constexpr int kLdsStride = kBK + 8;            // 16-bit elements per LDS row (80 bytes)
 constexpr int kChunkTiles = 8;                 // K tiles per accumulation chunk (256)
 
 // Round-to-nearest-even FP32 -> BF16 bits (finite inputs).
@@ -47,47 +53,120 @@ __device__ __forceinline__ uint32_t bf16_bits(float x)
 }
 __device__ __forceinline__ float bf16_val(uint32_t h) { return __uint_as_float(h << 16); }
 
-// Split x into npieces BF16 pieces (bits).
-template <int NP>
-__device__ __forceinline__ void split(float x, uint32_t (&p)[NP])
+// Pieces of one FP32 value.  BF16: x = hi + mid + lo + r, each piece the
+// round-to-nearest BF16 of what is left.  FP16 (scaled): with x' = x * 2^e
+// (exact), hi = fp16(x'), lo = fp16((x' - hi) * 2^11); x' - hi is exact in
+// FP32 and the pair carries 22 significant bits.
+template <int NP, bool FP16>
+__device__ __forceinline__ void split(float x, float scale, uint32_t (&p)[NP])
 {
-    float r = x;
+    if (FP16) {
+        const float xs = x * scale;
+        const _Float16 hi = (_Float16)xs;
+        const float r = xs - (float)hi;
+        const _Float16 lo = (_Float16)(r * 2048.0f);
+        p[0] = __builtin_bit_cast(uint16_t, hi);
+        if (NP > 1) p[NP > 1 ? 1 : 0] = __builtin_bit_cast(uint16_t, lo);
+    } else {
+        float r = x;
 #pragma unroll
-    for (int i = 0; i < NP; ++i) {
-        p[i] = bf16_bits(r);
-        r = r - bf16_val(p[i]);  // exact: r and its BF16 rounding agree in the top bits
+        for (int i = 0; i < NP; ++i) {
+            p[i] = bf16_bits(r);
+            r = r - bf16_val(p[i]);
+        }
     }
+}
+
+// Power-of-two row scales for the FP16 split: 2^e with e chosen so that the
+// row's largest magnitude lands in [2^14, 2^15) (below the FP16 maximum
+// 65504); inv = 2^-e undoes it in the epilogue.
+__device__ __forceinline__ void store_scale(float amax, int64_t r, float *scale, float *inv)
+{
+    int e = 0;
+    if (amax > 0.0f && amax < INFINITY) {
+        int p;
+        frexpf(amax, &p);  // amax = m * 2^p, m in [0.5, 1)
+        e = 15 - p;
+        e = e < -120 ? -120 : (e > 120 ? 120 : e);
+    }
+    scale[r] = ldexpf(1.0f, e);
+    inv[r] = ldexpf(1.0f, -e);
+}
+
+// X stored [R][K]: one workgroup per row (rows R .. Rp-1 get scale 1).
+__global__ void __launch_bounds__(256) rowscale_rows_kernel(const float *__restrict__ X, int64_t ld, int64_t R,
+                                                            int64_t K, float *__restrict__ scale,
+                                                            float *__restrict__ inv)
+{
+    __shared__ float part[256];
+    const int64_t r = blockIdx.x;
+    float amax = 0.0f;
+    if (r < R)
+        for (int64_t k = threadIdx.x; k < K; k += 256) amax = fmaxf(amax, fabsf(X[r * ld + k]));
+    part[threadIdx.x] = amax;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] = fmaxf(part[threadIdx.x], part[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) store_scale(part[0], r, scale, inv);
+}
+
+// X stored [K][R] (row r of op(X) is column r): partial maxima of each column
+// over K slices (blockIdx.y), coalesced across the columns of a workgroup.
+__global__ void __launch_bounds__(256) colmax_kernel(const float *__restrict__ X, int64_t ld, int64_t R,
+                                                     int64_t K, int64_t kslice, float *__restrict__ part,
+                                                     int64_t Rp)
+{
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= Rp) return;
+    const int64_t k0 = (int64_t)blockIdx.y * kslice, k1 = min(K, k0 + kslice);
+    float amax = 0.0f;
+    if (r < R)
+        for (int64_t k = k0; k < k1; ++k) amax = fmaxf(amax, fabsf(X[k * ld + r]));
+    part[(int64_t)blockIdx.y * Rp + r] = amax;
+}
+
+__global__ void __launch_bounds__(256) colscale_kernel(const float *__restrict__ part, int nslices, int64_t Rp,
+                                                       float *__restrict__ scale, float *__restrict__ inv)
+{
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= Rp) return;
+    float amax = 0.0f;
+    for (int q = 0; q < nslices; ++q) amax = fmaxf(amax, part[(int64_t)q * Rp + r]);
+    store_scale(amax, r, scale, inv);
 }
 
 // One 64 x 64 tile of op(X) (R logical rows x K) into NP planes [Rp][Kp].
 // KC: X is stored [R][K] (k contiguous); else [K][R].
-template <int NP, bool KC>
+template <int NP, bool FP16, bool KC>
 __global__ void __launch_bounds__(256) split_kernel(const float *__restrict__ X, int64_t ld, int64_t R,
                                                    int64_t K, uint16_t *__restrict__ planes,
-                                                   int64_t Kp, int64_t plane_elems)
+                                                   int64_t Kp, int64_t plane_elems,
+                                                   const float *__restrict__ scale)
 {
     __shared__ float tile[64][65];
     const int tid = threadIdx.x;
     const int64_t k0 = (int64_t)blockIdx.x * 64, r0 = (int64_t)blockIdx.y * 64;
     if (KC) {
-        // direct: thread covers (row, 4 consecutive k) pieces
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int e = tid + i * 256;  // 1024 groups of 4
             const int row = e / 16, kq = (e % 16) * 4;
             const int64_t gr = r0 + row;
+            const float sc = FP16 ? scale[gr] : 1.0f;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int64_t gk = k0 + kq + j;
                 const float x = (gr < R && gk < K) ? X[gr * ld + gk] : 0.0f;
                 uint32_t p[NP];
-                split<NP>(x, p);
+                split<NP, FP16>(x, sc, p);
 #pragma unroll
                 for (int q = 0; q < NP; ++q) planes[q * plane_elems + gr * Kp + gk] = (uint16_t)p[q];
             }
         }
     } else {
-        // transposed: coalesced reads along r into LDS, then writes along k
+        // coalesced reads along r into LDS, then writes along k
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             const int e = tid + i * 256;
@@ -100,45 +179,69 @@ __global__ void __launch_bounds__(256) split_kernel(const float *__restrict__ X,
         for (int i = 0; i < 16; ++i) {
             const int e = tid + i * 256;
             const int rr = e / 64, kk = e % 64;
-            const float x = tile[kk][rr];
-            uint32_t p[NP];
-            split<NP>(x, p);
             const int64_t gr = r0 + rr, gk = k0 + kk;
+            uint32_t p[NP];
+            split<NP, FP16>(tile[kk][rr], FP16 ? scale[gr] : 1.0f, p);
 #pragma unroll
             for (int q = 0; q < NP; ++q) planes[q * plane_elems + gr * Kp + gk] = (uint16_t)p[q];
         }
     }
 }
 
-// Product list: pairs (piece of A, piece of B), small terms first.
-template <int NTERMS>
-struct Terms;
+// Schemes: pieces per operand, products (piece of A, piece of B, accumulator),
+// small terms first, whether accumulation is two-level, waves per SIMD.
+enum { kBF16x6 = 0, kBF16x3 = 1, kFP16x3 = 2 };
+template <int S>
+struct Scheme;
 template <>
-struct Terms<6> {
-    static constexpr int NP = 3;
+struct Scheme<kBF16x6> {
+    static constexpr int NP = 3, NT = 6, NACC = 1, WPE = 1;
+    static constexpr bool FP16 = false, CHUNK = true;
     static constexpr int a[6] = {1, 0, 2, 0, 1, 0};
     static constexpr int b[6] = {1, 2, 0, 1, 0, 0};
+    static constexpr int acc[6] = {0, 0, 0, 0, 0, 0};
 };
 template <>
-struct Terms<3> {
-    static constexpr int NP = 2;
+struct Scheme<kBF16x3> {
+    static constexpr int NP = 2, NT = 3, NACC = 1, WPE = 2;
+    static constexpr bool FP16 = false, CHUNK = true;
     static constexpr int a[3] = {0, 1, 0};
     static constexpr int b[3] = {1, 0, 0};
+    static constexpr int acc[3] = {0, 0, 0};
+};
+template <>
+struct Scheme<kFP16x3> {
+    // hi*hi into acc 0; the cross terms (carrying 2^11) into acc 1
+    static constexpr int NP = 2, NT = 3, NACC = 2, WPE = 2;
+    static constexpr bool FP16 = true, CHUNK = false;
+    static constexpr int a[3] = {0, 1, 0};
+    static constexpr int b[3] = {1, 0, 0};
+    static constexpr int acc[3] = {1, 1, 0};
 };
 
-// f32 16x16 output of the bf16 MFMA: lane group g, register r hold row 4 g + r.
+template <bool FP16>
+__device__ __forceinline__ f4 mfma16(u32x4 a, u32x4 b, f4 c)
+{
+    if (FP16)
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b),
+                                                      c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                   0, 0, 0);
+}
+
+// f32 16x16 output of the 16-bit MFMAs: lane group g, register r hold row 4 g + r.
 __device__ __forceinline__ int out_row(int g, int r) { return 4 * g + r; }
 
-template <int NTERMS>
-__global__ void __launch_bounds__(256) planes_kernel(const uint16_t *__restrict__ Ap,
-                                                     const uint16_t *__restrict__ Bp, int64_t Kp,
-                                                     int64_t a_plane, int64_t b_plane, int tiles_m,
-                                                     int tiles_n, float *__restrict__ C, int64_t ldc,
-                                                     int64_t M, int64_t N, float alpha, float beta)
+template <int S>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Scheme<S>::WPE, Scheme<S>::WPE)))
+planes_kernel(const uint16_t *__restrict__ Ap, const uint16_t *__restrict__ Bp, int64_t Kp, int64_t a_plane,
+              int64_t b_plane, int tiles_m, int tiles_n, float *__restrict__ C, int64_t ldc, int64_t M,
+              int64_t N, float alpha, float beta, const float *__restrict__ a_inv,
+              const float *__restrict__ b_inv)
 {
-    using Tm = Terms<NTERMS>;
-    constexpr int NP = Tm::NP;
-    constexpr int PLANE_LDS = kBM * kLdsStride;           // bf16 per plane per stage
+    using Sc = Scheme<S>;
+    constexpr int NP = Sc::NP;
+    constexpr int PLANE_LDS = kBM * kLdsStride;           // 16-bit elements per plane per stage
     __shared__ __attribute__((aligned(16))) uint16_t lds[2][2][NP][PLANE_LDS];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -155,7 +258,7 @@ __global__ void __launch_bounds__(256) planes_kernel(const uint16_t *__restrict_
     const int tm = first_m + (pid % width) % gsz, tn = (pid % width) / gsz;
     const int64_t m0 = (int64_t)tm * kBM, n0 = (int64_t)tn * kBN;
 
-    // staging: per plane and operand 128 rows x 32 bf16 = 512 16-byte vectors, 2 per thread
+    // staging: per plane and operand 128 rows x 32 values = 512 16-byte vectors, 2 per thread
     u32x4 ra[NP][2], rb[NP][2];
     auto gload = [&](int64_t k0) {
 #pragma unroll
@@ -178,11 +281,19 @@ __global__ void __launch_bounds__(256) planes_kernel(const uint16_t *__restrict_
             }
     };
 
-    f4 acc[4][4], part[4][4];
+    f4 acc[Sc::NACC][4][4];
+    f4 part[Sc::CHUNK ? 4 : 1][Sc::CHUNK ? 4 : 1];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int q = 0; q < Sc::NACC; ++q)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = part[i][j] = f4{0, 0, 0, 0};
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[q][i][j] = f4{0, 0, 0, 0};
+    if (Sc::CHUNK)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) part[Sc::CHUNK ? i : 0][Sc::CHUNK ? j : 0] = f4{0, 0, 0, 0};
 
     const int nk = (int)(Kp / kBK);
     gload(0);
@@ -193,33 +304,35 @@ __global__ void __launch_bounds__(256) planes_kernel(const uint16_t *__restrict_
         const bool more = kt + 1 < nk;
         if (more) gload((int64_t)(kt + 1) * kBK);
 
-        bf16x8 fa[NP][4], fb[NP][4];
+        u32x4 fa[NP][4], fb[NP][4];
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-                fa[p][i] = *reinterpret_cast<const bf16x8 *>(
-                    &lds[cur][0][p][(wm + i * 16 + lr) * kLdsStride + lg * 8]);
+                fa[p][i] = *reinterpret_cast<const u32x4 *>(&lds[cur][0][p][(wm + i * 16 + lr) * kLdsStride + lg * 8]);
 #pragma unroll
             for (int j = 0; j < 4; ++j)
-                fb[p][j] = *reinterpret_cast<const bf16x8 *>(
-                    &lds[cur][1][p][(wn + j * 16 + lr) * kLdsStride + lg * 8]);
+                fb[p][j] = *reinterpret_cast<const u32x4 *>(&lds[cur][1][p][(wn + j * 16 + lr) * kLdsStride + lg * 8]);
         }
 #pragma unroll
-        for (int t = 0; t < NTERMS; ++t)
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    part[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[Tm::a[t]][i], fb[Tm::b[t]][j],
-                                                                        part[i][j], 0, 0, 0);
-        if ((kt + 1) % kChunkTiles == 0 || !more) {
+        for (int t = 0; t < Sc::NT; ++t)
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    acc[i][j] += part[i][j];
-                    part[i][j] = f4{0, 0, 0, 0};
+                    if (Sc::CHUNK)
+                        part[Sc::CHUNK ? i : 0][Sc::CHUNK ? j : 0] =
+                            mfma16<Sc::FP16>(fa[Sc::a[t]][i], fb[Sc::b[t]][j], part[Sc::CHUNK ? i : 0][Sc::CHUNK ? j : 0]);
+                    else
+                        acc[Sc::acc[t]][i][j] = mfma16<Sc::FP16>(fa[Sc::a[t]][i], fb[Sc::b[t]][j], acc[Sc::acc[t]][i][j]);
+                }
+        if (Sc::CHUNK && ((kt + 1) % kChunkTiles == 0 || !more)) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    acc[0][i][j] += part[Sc::CHUNK ? i : 0][Sc::CHUNK ? j : 0];
+                    part[Sc::CHUNK ? i : 0][Sc::CHUNK ? j : 0] = f4{0, 0, 0, 0};
                 }
         }
         if (more) lstore(cur ^ 1);
@@ -235,8 +348,11 @@ __global__ void __launch_bounds__(256) planes_kernel(const uint16_t *__restrict_
             for (int r = 0; r < 4; ++r) {
                 const int64_t row = m0 + wm + i * 16 + out_row(lg, r);
                 if (row < M && col < N) {
+                    float v = acc[0][i][j][r];
+                    if (Sc::NACC > 1) v += 0x1.0p-11f * acc[Sc::NACC - 1][i][j][r];
+                    if (Sc::FP16) v = v * a_inv[row] * b_inv[col];
                     float *p = C + row * ldc + col;
-                    float v = alpha * acc[i][j][r];
+                    v *= alpha;
                     if (beta != 0.0f) v += beta * *p;
                     *p = v;
                 }
@@ -246,55 +362,108 @@ __global__ void __launch_bounds__(256) planes_kernel(const uint16_t *__restrict_
 
 int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
-template <int NP>
+template <int NP, bool FP16>
 hipError_t launch_split(const float *X, int64_t ld, bool kc, int64_t R, int64_t K, uint16_t *planes,
-                        int64_t Rp, int64_t Kp, hipStream_t s)
+                        int64_t Rp, int64_t Kp, const float *scale, hipStream_t s)
 {
     const dim3 grid((unsigned)(Kp / 64), (unsigned)(Rp / 64));
     if (kc)
-        hipLaunchKernelGGL((split_kernel<NP, true>), grid, dim3(256), 0, s, X, ld, R, K, planes, Kp, Rp * Kp);
+        hipLaunchKernelGGL((split_kernel<NP, FP16, true>), grid, dim3(256), 0, s, X, ld, R, K, planes, Kp, Rp * Kp,
+                           scale);
     else
-        hipLaunchKernelGGL((split_kernel<NP, false>), grid, dim3(256), 0, s, X, ld, R, K, planes, Kp, Rp * Kp);
+        hipLaunchKernelGGL((split_kernel<NP, FP16, false>), grid, dim3(256), 0, s, X, ld, R, K, planes, Kp, Rp * Kp,
+                           scale);
+    return hipGetLastError();
+}
+
+constexpr int kScaleSlices = 64;  // K slices of the column-maximum pass
+
+hipError_t launch_rowscale(const float *X, int64_t ld, bool kc, int64_t R, int64_t K, float *scale, float *inv,
+                           int64_t Rp, float *part, hipStream_t s)
+{
+    if (kc) {
+        hipLaunchKernelGGL(rowscale_rows_kernel, dim3((unsigned)Rp), dim3(256), 0, s, X, ld, R, K, scale, inv);
+        return hipGetLastError();
+    }
+    const int64_t kslice = (K + kScaleSlices - 1) / kScaleSlices;
+    const int nslices = (int)((K + kslice - 1) / kslice);
+    const unsigned bx = (unsigned)((Rp + 255) / 256);
+    hipLaunchKernelGGL(colmax_kernel, dim3(bx, nslices), dim3(256), 0, s, X, ld, R, K, kslice, part, Rp);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(colscale_kernel, dim3(bx), dim3(256), 0, s, part, nslices, Rp, scale, inv);
+    return hipGetLastError();
+}
+
+struct Layout {
+    int64_t Mp, Np, Kp;
+    int np;
+    uint16_t *Ap, *Bp;
+    float *a_scale, *a_inv, *b_scale, *b_inv;
+    float *part;  // column-maximum partials, kScaleSlices x max(Mp, Np)
+};
+
+Layout layout(int64_t M, int64_t N, int64_t K, int np, bool scaled, void *work)
+{
+    Layout L;
+    L.Mp = round_up(M, kRowPad);
+    L.Np = round_up(N, kRowPad);
+    L.Kp = round_up(K, kKpad);
+    L.np = np;
+    L.Ap = (uint16_t *)work;
+    L.Bp = L.Ap + (size_t)np * L.Mp * L.Kp;
+    float *f = (float *)(L.Bp + (size_t)np * L.Np * L.Kp);
+    L.a_scale = scaled ? f : nullptr;
+    L.a_inv = scaled ? f + L.Mp : nullptr;
+    L.b_scale = scaled ? f + 2 * L.Mp : nullptr;
+    L.b_inv = scaled ? f + 2 * L.Mp + L.Np : nullptr;
+    L.part = scaled ? f + 2 * (L.Mp + L.Np) : nullptr;
+    return L;
+}
+
+template <int S>
+hipError_t run_scheme(const Gemm<float> &g, void *work, hipStream_t s)
+{
+    using Sc = Scheme<S>;
+    const Layout L = layout(g.M, g.N, g.K, Sc::NP, Sc::FP16, work);
+    hipError_t e = hipSuccess;
+    // op(A) rows are M x K: k contiguous unless A is transposed.
+    // op(B)^T rows are N x K: k contiguous when B is transposed.
+    if (Sc::FP16) {
+        e = launch_rowscale(g.A, g.lda, !g.ta, g.M, g.K, L.a_scale, L.a_inv, L.Mp, L.part, s);
+        if (e == hipSuccess) e = launch_rowscale(g.B, g.ldb, g.tb, g.N, g.K, L.b_scale, L.b_inv, L.Np, L.part, s);
+    }
+    if (e == hipSuccess) e = launch_split<Sc::NP, Sc::FP16>(g.A, g.lda, !g.ta, g.M, g.K, L.Ap, L.Mp, L.Kp, L.a_scale, s);
+    if (e == hipSuccess) e = launch_split<Sc::NP, Sc::FP16>(g.B, g.ldb, g.tb, g.N, g.K, L.Bp, L.Np, L.Kp, L.b_scale, s);
+    if (e != hipSuccess) return e;
+    const int64_t tiles_m = L.Mp / kBM, tiles_n = L.Np / kBN;
+    const int64_t nwg = tiles_m * tiles_n;
+    if (nwg > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(planes_kernel<S>, dim3((unsigned)nwg), dim3(256), 0, s, L.Ap, L.Bp, L.Kp, L.Mp * L.Kp,
+                       L.Np * L.Kp, (int)tiles_m, (int)tiles_n, g.C, g.ldc, g.M, g.N, g.alpha, g.beta, L.a_inv,
+                       L.b_inv);
     return hipGetLastError();
 }
 
 }  // namespace
 
-size_t split_bf16_workspace_bytes(int64_t M, int64_t N, int64_t K, int nterms)
+size_t split_workspace_bytes(int64_t M, int64_t N, int64_t K, int scheme)
 {
-    const int np = nterms == 6 ? 3 : 2;
+    const int np = scheme == kBF16x6 ? 3 : 2;
+    const bool scaled = scheme == kFP16x3;
     const int64_t Mp = round_up(M, kRowPad), Np = round_up(N, kRowPad), Kp = round_up(K, kKpad);
-    return (size_t)np * (size_t)(Mp + Np) * (size_t)Kp * sizeof(uint16_t);
+    return (size_t)np * (size_t)(Mp + Np) * (size_t)Kp * sizeof(uint16_t) +
+           (scaled ? ((size_t)2 * (Mp + Np) + (size_t)kScaleSlices * (Mp > Np ? Mp : Np)) * sizeof(float) : 0);
 }
 
-hipError_t launch_split_bf16(const Gemm<float> &g, int nterms, void *work, hipStream_t s)
+hipError_t launch_split_gemm(const Gemm<float> &g, int scheme, void *work, hipStream_t s)
 {
-    if (nterms != 3 && nterms != 6) return hipErrorInvalidValue;
-    const int64_t Mp = round_up(g.M, kRowPad), Np = round_up(g.N, kRowPad), Kp = round_up(g.K, kKpad);
-    const int np = nterms == 6 ? 3 : 2;
-    uint16_t *Ap = (uint16_t *)work;
-    uint16_t *Bp = Ap + (size_t)np * Mp * Kp;
-    hipError_t e;
-    // op(A) rows are M x K: k contiguous unless A is transposed.
-    // op(B)^T rows are N x K: k contiguous when B is transposed.
-    if (np == 3) {
-        e = launch_split<3>(g.A, g.lda, !g.ta, g.M, g.K, Ap, Mp, Kp, s);
-        if (e == hipSuccess) e = launch_split<3>(g.B, g.ldb, g.tb, g.N, g.K, Bp, Np, Kp, s);
-    } else {
-        e = launch_split<2>(g.A, g.lda, !g.ta, g.M, g.K, Ap, Mp, Kp, s);
-        if (e == hipSuccess) e = launch_split<2>(g.B, g.ldb, g.tb, g.N, g.K, Bp, Np, Kp, s);
+    switch (scheme) {
+    case kBF16x6: return run_scheme<kBF16x6>(g, work, s);
+    case kBF16x3: return run_scheme<kBF16x3>(g, work, s);
+    case kFP16x3: return run_scheme<kFP16x3>(g, work, s);
+    default: return hipErrorInvalidValue;
     }
-    if (e != hipSuccess) return e;
-    const int64_t tiles_m = Mp / kBM, tiles_n = Np / kBN;
-    const int64_t nwg = tiles_m * tiles_n;
-    if (nwg > 0x7fffffffLL) return hipErrorInvalidValue;
-    if (nterms == 6)
-        hipLaunchKernelGGL(planes_kernel<6>, dim3((unsigned)nwg), dim3(256), 0, s, Ap, Bp, Kp, Mp * Kp,
-                           Np * Kp, (int)tiles_m, (int)tiles_n, g.C, g.ldc, g.M, g.N, g.alpha, g.beta);
-    else
-        hipLaunchKernelGGL(planes_kernel<3>, dim3((unsigned)nwg), dim3(256), 0, s, Ap, Bp, Kp, Mp * Kp,
-                           Np * Kp, (int)tiles_m, (int)tiles_n, g.C, g.ldc, g.M, g.N, g.alpha, g.beta);
-    return hipGetLastError();
 }
 
 }  // namespace blasdsp

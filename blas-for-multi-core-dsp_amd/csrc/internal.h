@@ -40,10 +40,11 @@ hipError_t launch_skinny(const Gemm<T> &g, int splitk, T *work, hipStream_t s);
 template <typename T>
 int skinny_pick_splitk(const Gemm<T> &g, int num_cus);
 
-// Split-BF16 FP32 GEMM (gemm_split.hip).  `work` must hold
-// split_bf16_workspace_bytes(M, N, K, nterms) bytes; nterms is 3 or 6.
-size_t split_bf16_workspace_bytes(int64_t M, int64_t N, int64_t K, int nterms);
-hipError_t launch_split_bf16(const Gemm<float> &g, int nterms, void *work, hipStream_t s);
+// FP32 GEMM on 16-bit matrix cores (gemm_split.hip).  scheme: 0 = BF16 pieces
+// hi/mid/lo with 6 products, 1 = BF16 hi/lo with 3 products, 2 = scaled FP16
+// hi/lo with 3 products.  `work` must hold split_workspace_bytes(...) bytes.
+size_t split_workspace_bytes(int64_t M, int64_t N, int64_t K, int scheme);
+hipError_t launch_split_gemm(const Gemm<float> &g, int scheme, void *work, hipStream_t s);
 
 // Synthetic input generator (fill.hip).
 template <typename T>

@@ -179,6 +179,16 @@ BLASDSP_API int blasdsp_sgemm_split_bf16(char transA, char transB, int64_t M, in
                                          float alpha, const float *A, int64_t lda,
                                          const float *B, int64_t ldb, float beta,
                                          float *C, int64_t ldc, int nterms, void *stream);
+/* FP32 GEMM on the FP16 matrix cores: every row of op(A) and column of op(B)
+ * is scaled by a power of two (largest magnitude into [2^14, 2^15)), split
+ * into FP16 hi + 2^-11 lo (22 significant bits), and the three products
+ * hi*hi + 2^-11 (hi*lo + lo*hi) are accumulated in FP32.  Elements more than
+ * 2^29 below their row's (column's) largest magnitude lose precision to FP16
+ * subnormals.  Same semantics and arguments as blasdsp_sgemm. */
+BLASDSP_API int blasdsp_sgemm_split_fp16(char transA, char transB, int64_t M, int64_t N, int64_t K,
+                                         float alpha, const float *A, int64_t lda,
+                                         const float *B, int64_t ldb, float beta,
+                                         float *C, int64_t ldc, void *stream);
 
 /* ---- grouped execution (no counterpart in the reference) -----------------
  * Many GEMMs of uneven shapes, transpose modes and precisions in one
