@@ -340,13 +340,21 @@ def bench_copy(args):
     stream = torch.cuda.current_stream()
     ms = _time_events(lambda: dst.copy_(src), max(3, min(args.steps, 20)), stream)
     gbs = 2 * 4 * n / ms / 1e6
-    # read-only stream: a sum over 4 GiB
+    # read-only stream of 4 GiB through the library's read probe, several grid sizes
+    import __graft_entry__
+
+    blas = __graft_entry__.load_package()
     big = torch.ones(2 * n, dtype=torch.float32, device="cuda")
-    ms_read = _time_events(lambda: big.sum(), max(3, min(args.steps, 20)), stream)
+    sink = torch.empty(1 << 16, dtype=torch.int32, device="cuda")
+    read_gbs = {}
+    for blocks in (1024, 2048, 4096, 8192, 16384):
+        ms_r = _time_events(lambda: blas.check("blasdsp_probe_read", blas.lib().blasdsp_probe_read(
+            big.data_ptr(), 8 * n, sink.data_ptr(), blocks, stream.cuda_stream)), max(3, min(args.steps, 20)), stream)
+        read_gbs[blocks] = round(8 * n / ms_r / 1e6, 1)
     out = _common(args, round(gbs, 1), "GB/s", "f32",
                   {"workload": "device copy of 2 GiB (read 2 GiB + write 2 GiB per launch)",
                    "bytes_read": 4 * n, "bytes_written": 4 * n, "ms": round(ms, 4),
-                   "read_only_sum_4gib_gbs": round(8 * n / ms_read / 1e6, 1)},
+                   "read_probe_4gib_gbs_by_grid": read_gbs},
                   {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                    "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None})
     print(json.dumps(out), flush=True)

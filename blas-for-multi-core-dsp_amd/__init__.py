@@ -139,6 +139,7 @@ def _declare(L: ctypes.CDLL) -> None:
     L.blasdsp_tune_save.argtypes = [ctypes.c_char_p]
     L.blasdsp_tune_load.argtypes = [ctypes.c_char_p]
     L.blasdsp_tune_clear.argtypes = []
+    L.blasdsp_probe_read.argtypes = [_vp, _i64, _vp, ctypes.c_int, _vp]
     for extra in _EXTRA_DECLS:
         extra(L)
 

@@ -228,8 +228,17 @@ __device__ __forceinline__ void mma_block(typename SK<T>::acc_t (&acc)[MT][4],
             for (int t = 0; t < 4; ++t) acc[mt][t] = SK<T>::mma(b.sop(mt, j), b.lop(t, j), acc[mt][t]);
 }
 
+// Waves per SIMD the register allocation must allow: the kernel streams from
+// HBM, so more resident waves means more bytes in flight per CU.
+template <typename T, int MT>
+struct SkinnyOcc {
+    static constexpr int WPE = sizeof(T) == 8 ? (MT == 1 ? 3 : (MT == 2 ? 2 : 1)) : (MT == 1 ? 4 : (MT == 2 ? 3 : 2));
+};
+
 template <typename T, int MT, bool S_KC, bool L_KC, bool TRANS_OUT>
-__global__ void __launch_bounds__(256) skinny_kernel(SkinnyArgs<T> a)
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(SkinnyOcc<T, MT>::WPE, SkinnyOcc<T, MT>::WPE)))
+skinny_kernel(SkinnyArgs<T> a)
 {
     using acc_t = typename SK<T>::acc_t;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -371,9 +380,13 @@ void roles(const Gemm<T> &g, SkinnyArgs<T> &a, bool &s_kc, bool &l_kc, bool &tra
 template <typename T>
 int skinny_pick_splitk(const Gemm<T> &g, int num_cus)
 {
+    // One full round of resident workgroups: a workgroup is one wave per
+    // SIMD, so WPE workgroups fill a CU.
+    const int64_t ms = g.M <= g.N ? g.M : g.N;
     const int64_t nl = g.M <= g.N ? g.N : g.M;
+    const int wpe = ms <= 16 ? SkinnyOcc<T, 1>::WPE : (ms <= 32 ? SkinnyOcc<T, 2>::WPE : SkinnyOcc<T, 4>::WPE);
     const int64_t wgs = (nl + kWaves * kPanel - 1) / (kWaves * kPanel);
-    const int64_t want = 4LL * (num_cus > 0 ? num_cus : 256);
+    const int64_t want = (int64_t)wpe * (num_cus > 0 ? num_cus : 256);
     int64_t sk = (want + wgs - 1) / wgs;
     const int64_t max_by_k = g.K / 512 > 0 ? g.K / 512 : 1;  // slices of >= 512
     if (sk > max_by_k) sk = max_by_k;

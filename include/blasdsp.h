@@ -296,6 +296,13 @@ BLASDSP_API int blasdsp_tune_save(const char *path);
 BLASDSP_API int blasdsp_tune_load(const char *path);
 BLASDSP_API int blasdsp_tune_clear(void);
 
+/* ---- diagnostics ---------------------------------------------------------
+ * Streams `bytes` (a multiple of 16, 16-byte aligned) from device memory p
+ * with `blocks` workgroups and writes one word per workgroup to out (device,
+ * `blocks` words): the achievable HBM read bandwidth, the roof the
+ * bandwidth-bound skinny kernels are measured against. */
+BLASDSP_API int blasdsp_probe_read(const void *p, int64_t bytes, void *out, int blocks, void *stream);
+
 /* ---- synthetic inputs ----------------------------------------------------
  * Fill the logical rows x cols matrix stored at X (device memory, leading
  * dimension ld, transposed storage when trans != 0) from the counter-based
