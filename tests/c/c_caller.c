@@ -1,0 +1,307 @@
+/* c_caller.c — the library driven from plain C, the way the reference's host
+ * demos drive the DSP kernels (dgemm_release_V1.1/host_code/dgemm_demo.c:
+ * 58-158: allocate, fill, call by name, wait, compare with the host GEMM).
+ *
+ * Built with gcc against include/blasdsp.h and linked to libblasdsp.so and
+ * the system HIP runtime (no Python, no torch in the process), so it checks
+ * the C boundary as a C program sees it.  Results are compared with the CPU
+ * oracle (oracle/liboracle.so): normwise error <= 1e-13 for FP64,
+ * (sqrt(K) + 4) 2^-23 for FP32, 1e-6 for the split-FP16 form, and exact for
+ * the integer known-answer case and the uniform generator.
+ *
+ * Prints one line per check and exits 0 only if every check passed. */
+#define __HIP_PLATFORM_AMD__ 1
+#include <hip/hip_runtime_api.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "blasdsp.h"
+#include "oracle.h"
+
+static int g_failed = 0;
+
+#define HIP_OK(x)                                                                      \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess) {                                                        \
+            fprintf(stderr, "HIP error %d at %s:%d\n", (int)e_, __FILE__, __LINE__);   \
+            exit(3);                                                                   \
+        }                                                                              \
+    } while (0)
+
+static void report(const char *name, double err, double tol, int status)
+{
+    const int ok = status == BLASDSP_SUCCESS && err <= tol;
+    if (!ok) g_failed = 1;
+    printf("%s %-28s status=%d err=%.3e tol=%.3e\n", ok ? "ok  " : "FAIL", name, status, err, tol);
+    fflush(stdout);
+}
+
+static double tol32(int64_t K) { return (sqrt((double)(K > 1 ? K : 1)) + 4.0) * ldexp(1.0, -23); }
+
+/* Stored shape of op(X) with logical rows x cols. */
+static void stored(char t, int64_t rows, int64_t cols, int64_t *sr, int64_t *sc)
+{
+    *sr = t == 'T' ? cols : rows;
+    *sc = t == 'T' ? rows : cols;
+}
+
+typedef int (*dgemm_fn)(int64_t, int64_t, int64_t, int64_t, int64_t, int64_t, const double *, const double *,
+                        double *, double, double, void *);
+typedef int (*sgemm_fn)(int64_t, int64_t, int64_t, int64_t, int64_t, int64_t, const float *, const float *,
+                        float *, float, float, void *);
+
+/* One FP64 call through a reference-named entry point, padded leading
+ * dimensions, vs the extended-precision oracle. */
+static void check_d(const char *name, dgemm_fn fn, char ta, char tb, int64_t M, int64_t N, int64_t K,
+                    double alpha, double beta, int dist, uint64_t seed, double tol)
+{
+    int64_t ar, ac, br, bc;
+    stored(ta, M, K, &ar, &ac);
+    stored(tb, K, N, &br, &bc);
+    const int64_t lda = ac + 5, ldb = bc + 3, ldc = N + 7;
+    double *A = malloc(sizeof(double) * ar * lda), *B = malloc(sizeof(double) * br * ldb);
+    double *C = malloc(sizeof(double) * M * ldc), *R = malloc(sizeof(double) * M * ldc);
+    oracle_fill_f64(A, M, K, lda, ta == 'T', dist, seed, NAN);
+    oracle_fill_f64(B, K, N, ldb, tb == 'T', dist == ORACLE_PATTERN11 ? ORACLE_PATTERN7 : dist, seed + 1, NAN);
+    oracle_fill_f64(C, M, N, ldc, 0, dist, seed + 2, NAN);
+    memcpy(R, C, sizeof(double) * M * ldc);
+    oracle_gemm_xp_f64(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, R, ldc);
+    double *dA, *dB, *dC;
+    HIP_OK(hipMalloc((void **)&dA, sizeof(double) * ar * lda));
+    HIP_OK(hipMalloc((void **)&dB, sizeof(double) * br * ldb));
+    HIP_OK(hipMalloc((void **)&dC, sizeof(double) * M * ldc));
+    HIP_OK(hipMemcpy(dA, A, sizeof(double) * ar * lda, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(dB, B, sizeof(double) * br * ldb, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(dC, C, sizeof(double) * M * ldc, hipMemcpyHostToDevice));
+    const int st = fn(lda, ldb, ldc, M, N, K, dA, dB, dC, alpha, beta, NULL);
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(C, dC, sizeof(double) * M * ldc, hipMemcpyDeviceToHost));
+    double maxabs = 0.0;
+    double err = oracle_normwise_f64(M, N, C, ldc, R, ldc, &maxabs);
+    if (dist == ORACLE_PATTERN11) { /* exact: every product and sum is an integer below 2^53 */
+        err = 0.0;
+        for (int64_t i = 0; i < M; ++i)
+            for (int64_t j = 0; j < N; ++j)
+                if (C[i * ldc + j] != R[i * ldc + j]) err = 1.0;
+    }
+    report(name, err, tol, st);
+    HIP_OK(hipFree(dA));
+    HIP_OK(hipFree(dB));
+    HIP_OK(hipFree(dC));
+    free(A); free(B); free(C); free(R);
+}
+
+/* FP32: the oracle accumulates in extended precision from the FP32 inputs. */
+static void check_s(const char *name, sgemm_fn fn, int split_fp16, char ta, char tb, int64_t M, int64_t N,
+                    int64_t K, float alpha, float beta, uint64_t seed, double tol)
+{
+    int64_t ar, ac, br, bc;
+    stored(ta, M, K, &ar, &ac);
+    stored(tb, K, N, &br, &bc);
+    const int64_t lda = ac + 3, ldb = bc + 1, ldc = N + 2;
+    float *A = malloc(sizeof(float) * ar * lda), *B = malloc(sizeof(float) * br * ldb);
+    float *C = malloc(sizeof(float) * M * ldc);
+    double *R = malloc(sizeof(double) * M * N);
+    oracle_fill_f32(A, M, K, lda, ta == 'T', ORACLE_UNIFORM, seed, NAN);
+    oracle_fill_f32(B, K, N, ldb, tb == 'T', ORACLE_UNIFORM, seed + 1, NAN);
+    oracle_fill_f32(C, M, N, ldc, 0, ORACLE_UNIFORM, seed + 2, NAN);
+    oracle_gemm_xp_f32(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, R, N);
+    float *dA, *dB, *dC;
+    HIP_OK(hipMalloc((void **)&dA, sizeof(float) * ar * lda));
+    HIP_OK(hipMalloc((void **)&dB, sizeof(float) * br * ldb));
+    HIP_OK(hipMalloc((void **)&dC, sizeof(float) * M * ldc));
+    HIP_OK(hipMemcpy(dA, A, sizeof(float) * ar * lda, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(dB, B, sizeof(float) * br * ldb, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(dC, C, sizeof(float) * M * ldc, hipMemcpyHostToDevice));
+    const int st = split_fp16
+                       ? blasdsp_sgemm_split_fp16(ta, tb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, NULL)
+                       : fn(lda, ldb, ldc, M, N, K, dA, dB, dC, alpha, beta, NULL);
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(C, dC, sizeof(float) * M * ldc, hipMemcpyDeviceToHost));
+    double maxabs = 0.0;
+    report(name, oracle_normwise_f32(M, N, C, ldc, R, N, &maxabs), tol, st);
+    HIP_OK(hipFree(dA));
+    HIP_OK(hipFree(dB));
+    HIP_OK(hipFree(dC));
+    free(A); free(B); free(C); free(R);
+}
+
+/* Several FP32 and FP64 problems of different shapes and modes in one grouped
+ * launch, each compared with the oracle. */
+static void check_grouped(void)
+{
+    enum { NP = 6 };
+    const int64_t shp[NP][3] = {{64, 48, 100}, {200, 136, 300}, {17, 513, 77}, {130, 33, 260}, {256, 256, 64}, {9, 9, 9}};
+    const char modes[NP][2] = {{'N', 'N'}, {'N', 'T'}, {'T', 'N'}, {'T', 'T'}, {'N', 'N'}, {'T', 'T'}};
+    blasdsp_gemm_desc d[NP];
+    void *hA[NP], *hB[NP], *hC[NP], *dA[NP], *dB[NP], *dC[NP];
+    double *R[NP];
+    for (int p = 0; p < NP; ++p) {
+        const int f64 = p % 2;
+        const size_t s = f64 ? 8 : 4;
+        const int64_t M = shp[p][0], N = shp[p][1], K = shp[p][2];
+        const char ta = modes[p][0], tb = modes[p][1];
+        int64_t ar, ac, br, bc;
+        stored(ta, M, K, &ar, &ac);
+        stored(tb, K, N, &br, &bc);
+        hA[p] = malloc(s * ar * ac);
+        hB[p] = malloc(s * br * bc);
+        hC[p] = malloc(s * M * N);
+        R[p] = malloc(sizeof(double) * M * N);
+        if (f64) {
+            oracle_fill_f64(hA[p], M, K, ac, ta == 'T', ORACLE_UNIFORM, 60 + p, 0.0);
+            oracle_fill_f64(hB[p], K, N, bc, tb == 'T', ORACLE_UNIFORM, 70 + p, 0.0);
+            oracle_fill_f64(hC[p], M, N, N, 0, ORACLE_UNIFORM, 80 + p, 0.0);
+            memcpy(R[p], hC[p], sizeof(double) * M * N);
+            oracle_gemm_xp_f64(ta, tb, M, N, K, 1.25, hA[p], ac, hB[p], bc, -0.75, R[p], N);
+        } else {
+            oracle_fill_f32(hA[p], M, K, ac, ta == 'T', ORACLE_UNIFORM, 60 + p, 0.0f);
+            oracle_fill_f32(hB[p], K, N, bc, tb == 'T', ORACLE_UNIFORM, 70 + p, 0.0f);
+            oracle_fill_f32(hC[p], M, N, N, 0, ORACLE_UNIFORM, 80 + p, 0.0f);
+            oracle_gemm_xp_f32(ta, tb, M, N, K, 1.25, hA[p], ac, hB[p], bc, -0.75, hC[p], N, R[p], N);
+        }
+        HIP_OK(hipMalloc(&dA[p], s * ar * ac));
+        HIP_OK(hipMalloc(&dB[p], s * br * bc));
+        HIP_OK(hipMalloc(&dC[p], s * M * N));
+        HIP_OK(hipMemcpy(dA[p], hA[p], s * ar * ac, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(dB[p], hB[p], s * br * bc, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(dC[p], hC[p], s * M * N, hipMemcpyHostToDevice));
+        d[p] = (blasdsp_gemm_desc){f64 ? BLASDSP_FLOAT64 : BLASDSP_FLOAT32, ta, tb, M, N, K, 1.25, -0.75,
+                                   dA[p], ac, dB[p], bc, dC[p], N};
+    }
+    const int st = blasdsp_gemm_grouped(d, NP, NULL);
+    HIP_OK(hipDeviceSynchronize());
+    double worst = 0.0;
+    int bad = 0;
+    for (int p = 0; p < NP; ++p) {
+        const int f64 = p % 2;
+        const int64_t M = shp[p][0], N = shp[p][1], K = shp[p][2];
+        HIP_OK(hipMemcpy(hC[p], dC[p], (f64 ? 8 : 4) * M * N, hipMemcpyDeviceToHost));
+        double maxabs = 0.0;
+        const double err = f64 ? oracle_normwise_f64(M, N, hC[p], N, R[p], N, &maxabs)
+                               : oracle_normwise_f32(M, N, hC[p], N, R[p], N, &maxabs);
+        if (err > (f64 ? 1e-13 : tol32(K))) bad = 1;
+        if (err > worst) worst = err;
+        HIP_OK(hipFree(dA[p]));
+        HIP_OK(hipFree(dB[p]));
+        HIP_OK(hipFree(dC[p]));
+        free(hA[p]); free(hB[p]); free(hC[p]); free(R[p]);
+    }
+    report("blasdsp_gemm_grouped (6)", bad ? 1.0 : worst, tol32(512), st);
+}
+
+/* The device generator against the host one (uniform draws are bit-exact). */
+static void check_fill(void)
+{
+    /* logical 123 x 77 stored transposed: 77 stored rows of 123, ld 126 */
+    const int64_t rows = 123, cols = 77, ld = 126, n = cols * ld;
+    double *h = malloc(sizeof(double) * n), *g = malloc(sizeof(double) * n);
+    double *d;
+    HIP_OK(hipMalloc((void **)&d, sizeof(double) * n));
+    oracle_fill_f64(h, rows, cols, ld, 1, ORACLE_UNIFORM, 99, -7.0);
+    const int st = blasdsp_fill_f64(d, rows, cols, ld, 1, ORACLE_UNIFORM, 99, -7.0, NULL);
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(g, d, sizeof(double) * n, hipMemcpyDeviceToHost));
+    const int same = memcmp(h, g, sizeof(double) * n) == 0;
+    report("blasdsp_fill_f64 (trans)", same ? 0.0 : 1.0, 0.0, st);
+    HIP_OK(hipFree(d));
+    free(h); free(g);
+}
+
+/* The request loop from C: 40 mixed FP32/FP64 requests of every mode, served
+ * in grouped launches and again one launch per request, each output against
+ * the oracle on the operands the loop generates (stored A, B, C are draws of
+ * the seeds 3 seed, 3 seed + 1, 3 seed + 2). */
+static void check_serve(void)
+{
+    enum { NR = 40 };
+    blasdsp_request reqs[NR];
+    uint64_t x = 12345;
+    for (int i = 0; i < NR; ++i) {
+        int64_t dims[3];
+        for (int k = 0; k < 3; ++k) {
+            x = oracle_mix64(x + (uint64_t)k);
+            dims[k] = 4 + (int64_t)(x % 197);
+        }
+        if (i % 4 == 3) dims[0] = 8 + i % 25;
+        reqs[i] = (blasdsp_request){i % 3 == 0 ? BLASDSP_FLOAT64 : BLASDSP_FLOAT32, "NT"[i % 2], "NT"[(i / 2) % 2],
+                                    dims[0], dims[1], dims[2], 0.75, i % 5 == 0 ? 0.0 : 1.0,
+                                    (uint64_t)(2000 + i), 0.0};
+    }
+    for (int grouped = 1; grouped >= 0; --grouped) {
+        void *outs[NR];
+        for (int i = 0; i < NR; ++i) outs[i] = malloc((reqs[i].dtype == BLASDSP_FLOAT64 ? 8 : 4) * reqs[i].M * reqs[i].N);
+        blasdsp_request_result res[NR];
+        blasdsp_serve_stats stats;
+        const blasdsp_serve_config cfg = {1LL << 30, 16, 2, grouped, 0.0};
+        const int st = blasdsp_serve(reqs, NR, &cfg, res, &stats, outs);
+        double worst = 0.0;
+        int bad = stats.completed != NR;
+        for (int i = 0; i < NR && st == BLASDSP_SUCCESS; ++i) {
+            const blasdsp_request *r = &reqs[i];
+            const int f64 = r->dtype == BLASDSP_FLOAT64;
+            const char ta = r->transA, tb = r->transB;
+            int64_t ar, ac, br, bc;
+            stored(ta, r->M, r->K, &ar, &ac);
+            stored(tb, r->K, r->N, &br, &bc);
+            double *R = malloc(sizeof(double) * r->M * r->N);
+            double err, maxabs = 0.0;
+            if (f64) {
+                double *A = malloc(8 * ar * ac), *B = malloc(8 * br * bc);
+                oracle_fill_f64(A, ar, ac, ac, 0, ORACLE_UNIFORM, 3 * r->seed, 0.0);
+                oracle_fill_f64(B, br, bc, bc, 0, ORACLE_UNIFORM, 3 * r->seed + 1, 0.0);
+                oracle_fill_f64(R, r->M, r->N, r->N, 0, ORACLE_UNIFORM, 3 * r->seed + 2, 0.0);
+                oracle_gemm_xp_f64(ta, tb, r->M, r->N, r->K, r->alpha, A, ac, B, bc, r->beta, R, r->N);
+                err = oracle_normwise_f64(r->M, r->N, outs[i], r->N, R, r->N, &maxabs);
+                free(A); free(B);
+            } else {
+                float *A = malloc(4 * ar * ac), *B = malloc(4 * br * bc), *C = malloc(4 * r->M * r->N);
+                oracle_fill_f32(A, ar, ac, ac, 0, ORACLE_UNIFORM, 3 * r->seed, 0.0f);
+                oracle_fill_f32(B, br, bc, bc, 0, ORACLE_UNIFORM, 3 * r->seed + 1, 0.0f);
+                oracle_fill_f32(C, r->M, r->N, r->N, 0, ORACLE_UNIFORM, 3 * r->seed + 2, 0.0f);
+                oracle_gemm_xp_f32(ta, tb, r->M, r->N, r->K, r->alpha, A, ac, B, bc, r->beta, C, r->N, R, r->N);
+                err = oracle_normwise_f32(r->M, r->N, outs[i], r->N, R, r->N, &maxabs);
+                free(A); free(B); free(C);
+            }
+            if (res[i].status != BLASDSP_SUCCESS || err > (f64 ? 1e-13 : tol32(r->K))) bad = 1;
+            if (err > worst) worst = err;
+            free(R);
+        }
+        for (int i = 0; i < NR; ++i) free(outs[i]);
+        report(grouped ? "blasdsp_serve grouped (40)" : "blasdsp_serve per-request (40)", bad ? 1.0 : worst,
+               tol32(200), st);
+    }
+}
+
+int main(void)
+{
+    printf("library %s\n", blasdsp_version());
+    fflush(stdout);
+    const dgemm_fn dfn[4] = {dgemm_NN, dgemm_NT, dgemm_TN, dgemm_TT};
+    const sgemm_fn sfn[4] = {sgemm_NN, sgemm_NT, sgemm_TN, sgemm_TT};
+    const dgemm_fn dshort[4] = {irr_shortA_dgemm_NN, irr_shortA_dgemm_NT, irr_shortA_dgemm_TN, irr_shortA_dgemm_TT};
+    const sgemm_fn sthin[4] = {irr_thinB_sgemm_NN, irr_thinB_sgemm_NT, irr_thinB_sgemm_TN, irr_thinB_sgemm_TT};
+    const char *mname[4] = {"NN", "NT", "TN", "TT"};
+    char name[64];
+    for (int m = 0; m < 4; ++m) {
+        const char ta = mname[m][0], tb = mname[m][1];
+        snprintf(name, sizeof name, "dgemm_%s 200x136x300", mname[m]);
+        check_d(name, dfn[m], ta, tb, 200, 136, 300, 1.5, -0.5, ORACLE_UNIFORM, 1, 1e-13);
+        snprintf(name, sizeof name, "sgemm_%s 257x129x1000", mname[m]);
+        check_s(name, sfn[m], 0, ta, tb, 257, 129, 1000, 1.5f, -0.5f, 11, tol32(1000));
+        snprintf(name, sizeof name, "irr_shortA_dgemm_%s", mname[m]);
+        check_d(name, dshort[m], ta, tb, 16, 3000, 2048, 1.0, 1.0, ORACLE_UNIFORM, 21, 1e-13);
+        snprintf(name, sizeof name, "irr_thinB_sgemm_%s", mname[m]);
+        check_s(name, sthin[m], 0, ta, tb, 3000, 24, 3000, 1.0f, 1.0f, 31, tol32(3000));
+    }
+    check_d("dgemm_NN integer %11 x %7", dgemm_NN, 'N', 'N', 130, 140, 1500, 1.0, 0.0, ORACLE_PATTERN11, 0, 0.0);
+    check_s("split_fp16 NT 128x160x512", NULL, 1, 'N', 'T', 128, 160, 512, 1.0f, 0.5f, 41, 1e-6);
+    check_grouped();
+    check_fill();
+    check_serve();
+    printf("%s\n", g_failed ? "FAILED" : "PASSED");
+    return g_failed ? 1 : 0;
+}
