@@ -147,12 +147,28 @@ def _declare(L: ctypes.CDLL) -> None:
 _EXTRA_DECLS: list = []
 
 
+def _bind_hip_runtime() -> None:
+    """Load PyTorch's HIP runtime before the library when torch is installed.
+
+    PyTorch ships its own libamdhip64 (soname libamdhip64.so.7).  Loaded
+    first, it satisfies the library's libamdhip64.so.7 dependency, so one HIP
+    runtime serves the process.  Loaded after the library has pulled in the
+    system copy, torch's loader asks for "libamdhip64.so", finds no loaded
+    object of that name and maps a second HIP (and HSA) runtime beside the
+    first."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib() -> ctypes.CDLL:
     """Load lib/libblasdsp.so (raises if it has not been built)."""
     global _lib
     if _lib is None:
         if not LIB_PATH.exists():
             raise FileNotFoundError(f"{LIB_PATH} is not built; run make -C {HERE}")
+        _bind_hip_runtime()
         L = ctypes.CDLL(str(LIB_PATH))
         _declare(L)
         _lib = L

@@ -2,8 +2,12 @@
 and rejects bad arguments on the host (no GPU needed: argument checks run
 before any HIP call)."""
 import ctypes
+import json
+import pathlib
 
 import pytest
+
+ROOT = pathlib.Path(__file__).resolve().parent.parent
 
 
 def test_library_exports_every_declared_function(blas):
@@ -107,3 +111,19 @@ def test_tune_table_load_query_save(blas, tmp_path):
     assert L.blasdsp_tune_load(str(bad).encode()) == blas.ERR_INVALID
     assert L.blasdsp_tune_load(str(tmp_path / "missing.txt").encode()) == blas.ERR_INVALID
     L.blasdsp_tune_clear()
+
+
+def test_one_hip_runtime_per_process():
+    """Loading the library and then importing torch maps a single HIP runtime."""
+    import subprocess
+    import sys
+
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "import __graft_entry__\n"
+            "__graft_entry__.load_package().lib()\n"
+            "import torch\n"
+            "maps = open('/proc/self/maps').read().splitlines()\n"
+            "import json; print(json.dumps(sorted({l.split()[-1] for l in maps if 'libamdhip64' in l})))\n") % str(ROOT)
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, check=True, timeout=300)
+    libs = json.loads(out.stdout.strip().splitlines()[-1])
+    assert len(libs) == 1, libs
