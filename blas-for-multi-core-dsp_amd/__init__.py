@@ -126,6 +126,7 @@ def _declare(L: ctypes.CDLL) -> None:
     L.blasdsp_fill_f32.argtypes = [_vp, _i64, _i64, _i64, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _f, _vp]
     L.blasdsp_sgemm_split_bf16.argtypes = gen + [_f, _vp, _i64, _vp, _i64, _f, _vp, _i64, ctypes.c_int, _vp]
     L.blasdsp_sgemm_split_fp16.argtypes = gen + [_f, _vp, _i64, _vp, _i64, _f, _vp, _i64, _vp]
+    L.blasdsp_sgemm_fp8.argtypes = gen + [_f, _vp, _i64, _vp, _i64, _f, _vp, _i64, _vp]
     L.blasdsp_gemm_grouped.argtypes = [ctypes.POINTER(GemmDesc), ctypes.c_int, _vp]
     L.blasdsp_group_plan_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
     L.blasdsp_group_plan_run.argtypes = [ctypes.c_void_p, ctypes.POINTER(GemmDesc), ctypes.c_int, _vp]
@@ -310,6 +311,15 @@ def sgemm_split_fp16(transA: str, transB: str, A, B, C, alpha: float = 1.0, beta
     check("blasdsp_sgemm_split_fp16",
           lib().blasdsp_sgemm_split_fp16(transA.encode(), transB.encode(), M, N, K, alpha, A.data_ptr(), _ld(A),
                                          B.data_ptr(), _ld(B), beta, C.data_ptr(), _ld(C), _stream(stream)))
+
+
+def sgemm_fp8(transA: str, transB: str, A, B, C, alpha: float = 1.0, beta: float = 0.0, stream=None) -> None:
+    """Scaled-FP8 (E4M3, per-128-block scales) FP32 GEMM on torch float32 device tensors (in place on C)."""
+    M, N = C.shape
+    K = A.shape[0] if transA.upper() == "T" else A.shape[1]
+    check("blasdsp_sgemm_fp8",
+          lib().blasdsp_sgemm_fp8(transA.encode(), transB.encode(), M, N, K, alpha, A.data_ptr(), _ld(A),
+                                  B.data_ptr(), _ld(B), beta, C.data_ptr(), _ld(C), _stream(stream)))
 
 
 def fill(X, rows: int, cols: int, dist: int, seed: int, trans: bool = False, pad: float = 0.0, stream=None):

@@ -270,6 +270,8 @@ int sgemm_null(int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N, int6
 
 namespace blasdsp {
 namespace {
+constexpr int kSchemeFP8 = 100;  // split_entry: the scaled FP8 form rather than a split scheme
+
 int split_entry(char transA, char transB, int64_t M, int64_t N, int64_t K, float alpha, const float *A,
                 int64_t lda, const float *B, int64_t ldb, float beta, float *C, int64_t ldc, int scheme,
                 void *stream)
@@ -281,8 +283,10 @@ int split_entry(char transA, char transB, int64_t M, int64_t N, int64_t K, float
     hipStream_t s = (hipStream_t)stream;
     if (K == 0 || alpha == 0.0f) return hip_status(launch_scale(g, s));
     void *work = nullptr;
-    hipError_t e = workspace(s, split_workspace_bytes(M, N, K, scheme), &work);
+    const size_t bytes = scheme == kSchemeFP8 ? fp8_workspace_bytes(M, N, K) : split_workspace_bytes(M, N, K, scheme);
+    hipError_t e = workspace(s, bytes, &work);
     if (e != hipSuccess) return hip_status(e);
+    if (scheme == kSchemeFP8) return hip_status(launch_fp8_gemm(g, work, s));
     return hip_status(launch_split_gemm(g, scheme, work, s));
 }
 }  // namespace
@@ -304,6 +308,13 @@ int blasdsp_sgemm_split_fp16(char transA, char transB, int64_t M, int64_t N, int
                              float *C, int64_t ldc, void *stream)
 {
     return blasdsp::split_entry(transA, transB, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, 2, stream);
+}
+
+int blasdsp_sgemm_fp8(char transA, char transB, int64_t M, int64_t N, int64_t K, float alpha, const float *A,
+                      int64_t lda, const float *B, int64_t ldb, float beta, float *C, int64_t ldc, void *stream)
+{
+    return blasdsp::split_entry(transA, transB, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, blasdsp::kSchemeFP8,
+                                stream);
 }
 
 int blasdsp_fill_f64(double *X, int64_t rows, int64_t cols, int64_t ld, int trans, int dist,

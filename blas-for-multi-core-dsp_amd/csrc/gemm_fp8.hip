@@ -1,0 +1,312 @@
+// gemm_fp8.hip — FP32 GEMM through the FP8 (E4M3) matrix cores with one
+// power-of-two scale per 128-element block of every row of op(A) and every
+// column of op(B).  No counterpart in the reference (whose sgemm is FP32 FMA
+// throughout, sgemm_release_V1.3.2/device_code/sgemm_kernel.c:1384): a
+// reduced-precision form offered beside the plain and split FP32 kernels, with
+// its error reported rather than bounded.
+//
+// Two passes:
+//   quant_kc_kernel / quant_mc_kernel  read op(A) rows and op(B) columns (K-
+//       contiguous or not, whichever way they are stored), pick each 128-block's
+//       exponent e (the smallest with amax 2^-e <= 448), and write the E4M3
+//       codes of x 2^-e (round to nearest even) K-contiguously, zero-padded to
+//       128-multiples in every dimension, plus the biased exponents 127 + e
+//       (E8M0 scale bytes);
+//   fp8_gemm_kernel  128 x 128 output tiles, K steps of 128 = one
+//       v_mfma_scale_f32_16x16x128_f8f6f4 depth, so each step's scales are
+//       one byte per row / column, applied by the matrix core itself.
+//
+// MFMA layout, measured with a lane probe: A lane l holds 32 bytes of row
+// l % 16 at k = 32 (l / 16) .. +31 (B the same for column l % 16, the same k),
+// lane l's scale byte applies to row (column) l % 16 and k block l / 16, and
+// output register r of lane l is row 4 (l / 16) + r, column l % 16.
+#include <math.h>
+
+#include "api_common.h"
+
+namespace blasdsp {
+namespace {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef int i8v __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kQB = 128;         // k per scale block = MFMA depth = K step
+constexpr int kTile = 128;       // output tile edge
+constexpr int kRow = 144;        // LDS row stride in bytes: 128 data + 16 pad
+constexpr int kRowBlock = 64;    // rows whose four scales of one k block share a word
+
+// Round-to-nearest-even E4M3 (OCP: bias 7, subnormal step 2^-9, 0x7F NaN).
+// Inputs here satisfy |y| <= 448; non-finite inputs give NaN.
+__device__ __forceinline__ uint32_t e4m3(float y)
+{
+    const uint32_t sign = (__float_as_uint(y) >> 24) & 0x80u;
+    const float a = fabsf(y);
+    if (!(a < INFINITY)) return 0x7Fu;
+    if (a < 0x1p-6f) return sign | (uint32_t)rintf(a * 0x1p9f);
+    int x;
+    (void)frexpf(a, &x);
+    int E = x - 1;
+    float m = rintf(ldexpf(a, 3 - E));
+    if (m >= 16.0f) {
+        m = 8.0f;
+        E += 1;
+    }
+    if (E > 8 || (E == 8 && m > 14.0f)) return sign | 0x7Eu;
+    return sign | (uint32_t)((E + 7) << 3) | ((uint32_t)m - 8u);
+}
+
+// Smallest e with amax 2^-e <= 448, clamped to [-127, 127]; 0 for a zero block.
+__device__ __forceinline__ int block_exp(float amax)
+{
+    if (!(amax > 0.0f)) return 0;
+    int x;
+    const float f = frexpf(amax, &x);
+    const int e = x - 9 + (f > 0.875f ? 1 : 0);
+    return e < -127 ? -127 : (e > 127 ? 127 : e);
+}
+
+// Byte offset of the scale of row r (of Rp padded rows) in k block kb: the
+// four rows r0 + 16 i + l (i = 0..3) of a 64-row group are one 32-bit word,
+// so a lane of the GEMM reads its four fragments' scales with one load.
+__device__ __forceinline__ int64_t scale_byte(int64_t r, int64_t kb, int64_t Rp)
+{
+    return ((kb * (Rp / kRowBlock) + r / kRowBlock) * 16 + (r % 16)) * 4 + (r % kRowBlock) / 16;
+}
+
+// Rows of op(X) stored K-contiguously: element (r, k) at X[r ld + k].  One
+// wave per (row, k block), two elements per lane.
+__global__ void __launch_bounds__(256) quant_kc_kernel(const float *__restrict__ X, int64_t ld, int64_t R,
+                                                       int64_t K, int64_t Rp, int64_t KB,
+                                                       uint8_t *__restrict__ Q, uint8_t *__restrict__ S)
+{
+    const int64_t id = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (id >= Rp * KB) return;  // whole waves only; no barrier below
+    const int lane = threadIdx.x & 63;
+    const int64_t r = id / KB, kb = id % KB, k = kb * kQB + 2 * lane;
+    float x0 = 0.0f, x1 = 0.0f;
+    if (r < R) {
+        if (k < K) x0 = X[r * ld + k];
+        if (k + 1 < K) x1 = X[r * ld + k + 1];
+    }
+    float amax = fmaxf(fabsf(x0), fabsf(x1));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) amax = fmaxf(amax, __shfl_xor(amax, o));
+    const int e = block_exp(amax);
+    const uint32_t q = e4m3(ldexpf(x0, -e)) | (e4m3(ldexpf(x1, -e)) << 8);
+    *reinterpret_cast<uint16_t *>(Q + r * (KB * kQB) + k) = (uint16_t)q;
+    if (lane == 0) S[scale_byte(r, kb, Rp)] = (uint8_t)(127 + e);
+}
+
+// Rows of op(X) stored transposed: element (r, k) at X[k ld + r].  A
+// workgroup takes 64 rows x one k block through LDS: coalesced loads along
+// r, then each thread quantizes 32 consecutive k of one row.
+__global__ void __launch_bounds__(256) quant_mc_kernel(const float *__restrict__ X, int64_t ld, int64_t R,
+                                                       int64_t K, int64_t Rp, int64_t KB,
+                                                       uint8_t *__restrict__ Q, uint8_t *__restrict__ S)
+{
+    __shared__ float T[64][kQB + 1];
+    __shared__ float wmax[4][64];
+    const int tid = threadIdx.x, rl = tid & 63, w = tid >> 6;
+    const int64_t r = (int64_t)blockIdx.x * 64 + rl, kb = blockIdx.y, k0 = kb * kQB;
+#pragma unroll 8
+    for (int i = 0; i < 32; ++i) {
+        const int k = w + 4 * i;
+        float v = 0.0f;
+        if (r < R && k0 + k < K) v = X[(k0 + k) * ld + r];
+        T[rl][k] = v;
+    }
+    __syncthreads();
+    float amax = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 32; ++j) amax = fmaxf(amax, fabsf(T[rl][w * 32 + j]));
+    wmax[w][rl] = amax;
+    __syncthreads();
+    amax = fmaxf(fmaxf(wmax[0][rl], wmax[1][rl]), fmaxf(wmax[2][rl], wmax[3][rl]));
+    const int e = block_exp(amax);
+    u32x4 out[2];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        uint32_t word = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) word |= e4m3(ldexpf(T[rl][w * 32 + 4 * j + b], -e)) << (8 * b);
+        out[j / 4][j % 4] = word;
+    }
+    u32x4 *dst = reinterpret_cast<u32x4 *>(Q + r * (KB * kQB) + k0 + w * 32);
+    dst[0] = out[0];
+    dst[1] = out[1];
+    if (w == 0) S[scale_byte(r, kb, Rp)] = (uint8_t)(127 + e);
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+fp8_gemm_kernel(const uint8_t *__restrict__ Qa, const uint8_t *__restrict__ Qb, const uint32_t *__restrict__ Sa,
+                const uint32_t *__restrict__ Sb, int64_t Kp, int64_t Mp, int64_t Np, int tiles_m, int tiles_n,
+                float *__restrict__ C, int64_t ldc, int64_t M, int64_t N, float alpha, float beta)
+{
+    __shared__ __attribute__((aligned(16))) uint8_t lds[2][2][kTile * kRow];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lr = lane & 15, lg = lane >> 4;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+
+    // XCD-aware grouped tile order, as in the FP32/FP64 tile kernels.
+    int pid = blockIdx.x;
+    const int nwg = gridDim.x;
+    if ((nwg & 7) == 0) pid = (pid & 7) * (nwg >> 3) + (pid >> 3);
+    const int width = 8 * tiles_n;
+    const int first_m = (pid / width) * 8;
+    const int gsz = min(tiles_m - first_m, 8);
+    const int tm = first_m + (pid % width) % gsz, tn = (pid % width) / gsz;
+    const int64_t m0 = (int64_t)tm * kTile, n0 = (int64_t)tn * kTile;
+
+    // staging: 128 rows x 128 bytes per operand = 1024 16-byte vectors, 4 per thread
+    u32x4 ra[4], rb[4];
+    auto gload = [&](int64_t k0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int c = tid + i * 256, row = c >> 3, col = (c & 7) * 16;
+            ra[i] = *reinterpret_cast<const u32x4 *>(Qa + (m0 + row) * Kp + k0 + col);
+            rb[i] = *reinterpret_cast<const u32x4 *>(Qb + (n0 + row) * Kp + k0 + col);
+        }
+    };
+    auto lstore = [&](int st) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int c = tid + i * 256, row = c >> 3, col = (c & 7) * 16;
+            *reinterpret_cast<u32x4 *>(&lds[st][0][row * kRow + col]) = ra[i];
+            *reinterpret_cast<u32x4 *>(&lds[st][1][row * kRow + col]) = rb[i];
+        }
+    };
+    // scale words of this lane's four A rows and four B columns in k block kb
+    const int64_t a_grp = (m0 + wm) / kRowBlock, b_grp = (n0 + wn) / kRowBlock;
+    const int64_t a_groups = Mp / kRowBlock, b_groups = Np / kRowBlock;
+    auto sload = [&](int64_t kb, uint32_t &sa, uint32_t &sb) {
+        sa = Sa[(kb * a_groups + a_grp) * 16 + lr];
+        sb = Sb[(kb * b_groups + b_grp) * 16 + lr];
+    };
+
+    f4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f4{0, 0, 0, 0};
+
+    const int nk = (int)(Kp / kQB);
+    uint32_t sa, sb, sa_next = 0, sb_next = 0;
+    gload(0);
+    sload(0, sa, sb);
+    lstore(0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        const bool more = kt + 1 < nk;
+        if (more) {
+            gload((int64_t)(kt + 1) * kQB);
+            sload(kt + 1, sa_next, sb_next);
+        }
+        i8v fa[4], fb[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint8_t *pa = &lds[cur][0][(wm + i * 16 + lr) * kRow + lg * 32];
+            const uint8_t *pb = &lds[cur][1][(wn + i * 16 + lr) * kRow + lg * 32];
+            const u32x4 a0 = *reinterpret_cast<const u32x4 *>(pa), a1 = *reinterpret_cast<const u32x4 *>(pa + 16);
+            const u32x4 b0 = *reinterpret_cast<const u32x4 *>(pb), b1 = *reinterpret_cast<const u32x4 *>(pb + 16);
+            fa[i] = i8v{(int)a0[0], (int)a0[1], (int)a0[2], (int)a0[3], (int)a1[0], (int)a1[1], (int)a1[2], (int)a1[3]};
+            fb[i] = i8v{(int)b0[0], (int)b0[1], (int)b0[2], (int)b0[3], (int)b1[0], (int)b1[1], (int)b1[2], (int)b1[3]};
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int sai = (int)((sa >> (8 * i)) & 0xFFu);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int sbj = (int)((sb >> (8 * j)) & 0xFFu);
+                acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(fa[i], fb[j], acc[i][j], 0, 0, 0, sai,
+                                                                             0, sbj);
+            }
+        }
+        if (more) {
+            lstore(cur ^ 1);
+            sa = sa_next;
+            sb = sb_next;
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t row = m0 + wm + i * 16 + 4 * lg + r, col = n0 + wn + j * 16 + lr;
+                if (row < M && col < N) {
+                    float v = alpha * acc[i][j][r];
+                    if (beta != 0.0f) v += beta * C[row * ldc + col];
+                    C[row * ldc + col] = v;
+                }
+            }
+}
+
+struct Fp8Layout {
+    int64_t Mp, Np, Kp, KB;
+    uint8_t *Qa, *Qb, *Sa, *Sb;
+    size_t bytes;
+};
+
+size_t up256(size_t b) { return (b + 255) / 256 * 256; }
+
+Fp8Layout fp8_layout(int64_t M, int64_t N, int64_t K, void *base)
+{
+    Fp8Layout L;
+    L.Mp = (M + kTile - 1) / kTile * kTile;
+    L.Np = (N + kTile - 1) / kTile * kTile;
+    L.KB = (K + kQB - 1) / kQB;
+    L.Kp = L.KB * kQB;
+    uint8_t *p = (uint8_t *)base;
+    size_t off = 0;
+    L.Qa = p + off;
+    off += up256((size_t)(L.Mp * L.Kp));
+    L.Qb = p + off;
+    off += up256((size_t)(L.Np * L.Kp));
+    L.Sa = p + off;
+    off += up256((size_t)(L.Mp * L.KB));
+    L.Sb = p + off;
+    off += up256((size_t)(L.Np * L.KB));
+    L.bytes = off;
+    return L;
+}
+
+hipError_t quantize(const float *X, int64_t ld, bool k_contig, int64_t R, int64_t K, int64_t Rp, int64_t KB,
+                    uint8_t *Q, uint8_t *S, hipStream_t s)
+{
+    if (k_contig) {
+        const int64_t waves = Rp * KB;
+        hipLaunchKernelGGL(quant_kc_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, X, ld, R, K, Rp,
+                           KB, Q, S);
+    } else {
+        hipLaunchKernelGGL(quant_mc_kernel, dim3((unsigned)(Rp / 64), (unsigned)KB), dim3(256), 0, s, X, ld, R, K,
+                           Rp, KB, Q, S);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+size_t fp8_workspace_bytes(int64_t M, int64_t N, int64_t K) { return fp8_layout(M, N, K, nullptr).bytes; }
+
+hipError_t launch_fp8_gemm(const Gemm<float> &g, void *work, hipStream_t s)
+{
+    const Fp8Layout L = fp8_layout(g.M, g.N, g.K, work);
+    // op(A) rows are K-contiguous unless A is transposed; op(B) columns are
+    // K-contiguous exactly when B is transposed (stored N x K).
+    hipError_t e = quantize(g.A, g.lda, !g.ta, g.M, g.K, L.Mp, L.KB, L.Qa, L.Sa, s);
+    if (e == hipSuccess) e = quantize(g.B, g.ldb, g.tb, g.N, g.K, L.Np, L.KB, L.Qb, L.Sb, s);
+    if (e != hipSuccess) return e;
+    const int tiles_m = (int)(L.Mp / kTile), tiles_n = (int)(L.Np / kTile);
+    hipLaunchKernelGGL(fp8_gemm_kernel, dim3((unsigned)(tiles_m * tiles_n)), dim3(256), 0, s, L.Qa, L.Qb,
+                       (const uint32_t *)L.Sa, (const uint32_t *)L.Sb, L.Kp, L.Mp, L.Np, tiles_m, tiles_n, g.C,
+                       g.ldc, g.M, g.N, g.alpha, g.beta);
+    return hipGetLastError();
+}
+
+}  // namespace blasdsp

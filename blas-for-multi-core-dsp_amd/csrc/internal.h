@@ -46,6 +46,11 @@ int skinny_pick_splitk(const Gemm<T> &g, int num_cus);
 size_t split_workspace_bytes(int64_t M, int64_t N, int64_t K, int scheme);
 hipError_t launch_split_gemm(const Gemm<float> &g, int scheme, void *work, hipStream_t s);
 
+// FP32 GEMM on the FP8 (E4M3) matrix cores with per-128-block power-of-two
+// scales (gemm_fp8.hip).  `work` must hold fp8_workspace_bytes(...) bytes.
+size_t fp8_workspace_bytes(int64_t M, int64_t N, int64_t K);
+hipError_t launch_fp8_gemm(const Gemm<float> &g, void *work, hipStream_t s);
+
 // Synthetic input generator (fill.hip).
 template <typename T>
 hipError_t launch_fill(T *X, int64_t rows, int64_t cols, int64_t ld, int trans, int dist,

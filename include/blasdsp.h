@@ -190,6 +190,19 @@ BLASDSP_API int blasdsp_sgemm_split_fp16(char transA, char transB, int64_t M, in
                                          const float *B, int64_t ldb, float beta,
                                          float *C, int64_t ldc, void *stream);
 
+/* FP32 GEMM on the FP8 matrix cores (no counterpart in the reference): every
+ * row of op(A) and column of op(B) is cut into blocks of 128 consecutive k,
+ * each block gets the power-of-two scale 2^e with e the smallest exponent for
+ * which amax 2^-e <= 448, and its elements are stored as E4M3 (round to
+ * nearest even) of x 2^-e; the matrix cores apply the scales and accumulate
+ * in FP32.  Precision is E4M3's (3 fraction bits): a normwise error of a few
+ * 1e-2 on random inputs, reported, not bounded.  Non-finite inputs give NaN
+ * in their rows and columns.  Same semantics and arguments as blasdsp_sgemm. */
+BLASDSP_API int blasdsp_sgemm_fp8(char transA, char transB, int64_t M, int64_t N, int64_t K,
+                                  float alpha, const float *A, int64_t lda,
+                                  const float *B, int64_t ldb, float beta,
+                                  float *C, int64_t ldc, void *stream);
+
 /* ---- grouped execution (no counterpart in the reference) -----------------
  * Many GEMMs of uneven shapes, transpose modes and precisions in one
  * persistent launch per precision.  Each problem has the semantics of

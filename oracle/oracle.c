@@ -411,3 +411,122 @@ double oracle_normwise_f32(int64_t M, int64_t N, const float *X, int64_t ldx,
     if (den == 0.0L) return num == 0.0L ? 0.0 : INFINITY;
     return (double)sqrtl(num / den);
 }
+
+/* ---- scaled FP8 (E4M3) GEMM ------------------------------------------------
+ * No reference counterpart: the FP8 form of the FP32 GEMM.  The rule, shared
+ * with the library's quantizer:
+ *   - every row of op(A) and every column of op(B) is cut into blocks of 128
+ *     consecutive k (the last block zero-padded);
+ *   - a block with largest magnitude amax = f 2^x (f in [0.5, 1)) gets the
+ *     exponent e = x - 9 + (f > 0.875), the smallest e with amax 2^-e <= 448
+ *     (the largest finite E4M3 value), clamped to [-127, 127]; a zero block
+ *     gets e = 0;
+ *   - each element is stored as q = E4M3(x 2^-e), rounded to nearest even
+ *     (OCP E4M3: bias 7, subnormal step 2^-9, no infinities);
+ *   - C = alpha sum_blocks 2^(ea + eb) sum_k qa qb + beta C.
+ * Pinned in tests/test_oracle.py: the encoder against PyTorch's float8_e4m3fn
+ * conversion, the block exponent against its defining inequality, and the
+ * quantized product against exact rational arithmetic on small inputs. */
+
+unsigned oracle_fp8_e4m3(float y)
+{
+    unsigned sign = signbit(y) ? 0x80u : 0u;
+    float a = fabsf(y);
+    if (a != a) return 0x7Fu;
+    if (a < 0x1p-6f) {                      /* subnormal range: multiples of 2^-9 */
+        float m = rintf(a * 0x1p9f);        /* 0 .. 8; 8 is the smallest normal */
+        return sign | (unsigned)m;
+    }
+    int x;
+    (void)frexpf(a, &x);                    /* a = f 2^x, f in [0.5, 1) */
+    int E = x - 1;                          /* a in [2^E, 2^(E+1)) */
+    float m = rintf(ldexpf(a, 3 - E));      /* 8 .. 16 */
+    if (m >= 16.0f) {
+        m = 8.0f;
+        E += 1;
+    }
+    if (E > 8 || (E == 8 && m > 14.0f)) return sign | 0x7Eu; /* saturate to 448 */
+    return sign | (unsigned)((E + 7) << 3) | ((unsigned)m - 8u);
+}
+
+float oracle_fp8_decode(unsigned q)
+{
+    const float s = (q & 0x80u) ? -1.0f : 1.0f;
+    const unsigned e = (q >> 3) & 15u, m = q & 7u;
+    if (e == 15u && m == 7u) return NAN;
+    if (e == 0u) return s * ldexpf((float)m, -9);
+    return s * ldexpf((float)(8u + m), (int)e - 10);
+}
+
+int oracle_fp8_block_exp(float amax)
+{
+    if (!(amax > 0.0f)) return 0;
+    int x;
+    const float f = frexpf(amax, &x);
+    int e = x - 9 + (f > 0.875f ? 1 : 0);
+    if (e < -127) e = -127;
+    if (e > 127) e = 127;
+    return e;
+}
+
+/* op(X) row r (r < R), k-th element, X stored row-major with ld; trans: op(X) = X^T. */
+static float opx(const float *X, int64_t ld, int trans, int64_t r, int64_t k)
+{
+    return trans ? X[IDX(k, r, ld)] : X[IDX(r, k, ld)];
+}
+
+/* Quantize the R vectors of length K of op(X) (rows of op(A): trans = ta;
+ * columns of op(B): pass B with the opposite sense) into q[R][KB*128] codes
+ * decoded to float, and their block exponents e[R][KB]. */
+static void fp8_quantize(const float *X, int64_t ld, int trans, int64_t R, int64_t K, float *q, int *e)
+{
+    const int64_t KB = (K + 127) / 128;
+#pragma omp parallel for num_threads(oracle_get_threads()) schedule(static)
+    for (int64_t r = 0; r < R; r++)
+        for (int64_t b = 0; b < KB; b++) {
+            float amax = 0.0f;
+            for (int64_t k = b * 128; k < K && k < b * 128 + 128; k++) amax = fmaxf(amax, fabsf(opx(X, ld, trans, r, k)));
+            const int be = oracle_fp8_block_exp(amax);
+            e[IDX(r, b, KB)] = be;
+            for (int64_t k = b * 128; k < b * 128 + 128; k++) {
+                const float v = k < K ? ldexpf(opx(X, ld, trans, r, k), -be) : 0.0f;
+                q[IDX(r, k, KB * 128)] = oracle_fp8_decode(oracle_fp8_e4m3(v));
+            }
+        }
+}
+
+void oracle_sgemm_fp8(char transA, char transB, int64_t M, int64_t N, int64_t K, double alpha,
+                      const float *A, int64_t lda, const float *B, int64_t ldb, double beta,
+                      const float *C32, int64_t ldc32, double *C64, int64_t ldc64)
+{
+    const int ta = (transA == 'T' || transA == 't');
+    const int tb = (transB == 'T' || transB == 't');
+    const int64_t KB = (K + 127) / 128, Kp = KB * 128;
+    float *qa = (float *)malloc(sizeof(float) * (size_t)(M > 0 ? M : 1) * (size_t)(Kp > 0 ? Kp : 1));
+    float *qb = (float *)malloc(sizeof(float) * (size_t)(N > 0 ? N : 1) * (size_t)(Kp > 0 ? Kp : 1));
+    int *ea = (int *)malloc(sizeof(int) * (size_t)(M > 0 ? M : 1) * (size_t)(KB > 0 ? KB : 1));
+    int *eb = (int *)malloc(sizeof(int) * (size_t)(N > 0 ? N : 1) * (size_t)(KB > 0 ? KB : 1));
+    if (!qa || !qb || !ea || !eb) {
+        free(qa); free(qb); free(ea); free(eb);
+        return;
+    }
+    fp8_quantize(A, lda, ta, M, K, qa, ea);
+    /* column j of op(B): op(B)[k][j] = tb ? B[j][k] : B[k][j], i.e. row j of the
+     * transposed view, which is "trans" exactly when tb is false */
+    fp8_quantize(B, ldb, !tb, N, K, qb, eb);
+#pragma omp parallel for num_threads(oracle_get_threads()) schedule(static)
+    for (int64_t i = 0; i < M; i++)
+        for (int64_t j = 0; j < N; j++) {
+            long double acc = 0.0L;
+            for (int64_t b = 0; b < KB; b++) {
+                long double part = 0.0L;
+                for (int64_t k = b * 128; k < b * 128 + 128; k++)
+                    part += (long double)qa[IDX(i, k, Kp)] * (long double)qb[IDX(j, k, Kp)];
+                acc += ldexpl(part, ea[IDX(i, b, KB)] + eb[IDX(j, b, KB)]);
+            }
+            long double out = (long double)alpha * acc;
+            if (beta != 0.0) out += (long double)beta * (long double)C32[IDX(i, j, ldc32)];
+            C64[IDX(i, j, ldc64)] = (double)out;
+        }
+    free(qa); free(qb); free(ea); free(eb);
+}

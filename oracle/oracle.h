@@ -110,6 +110,16 @@ double oracle_normwise_f64(int64_t M, int64_t N, const double *X, int64_t ldx,
 double oracle_normwise_f32(int64_t M, int64_t N, const float *X, int64_t ldx,
                            const double *R, int64_t ldr, double *maxabs);
 
+/* Scaled FP8 (E4M3, one power-of-two scale per 128-element block of each row
+ * of op(A) and column of op(B)); see oracle.c for the rule.  No reference
+ * counterpart. */
+unsigned oracle_fp8_e4m3(float y);            /* RNE encode, |y| <= 448 */
+float oracle_fp8_decode(unsigned q);
+int oracle_fp8_block_exp(float amax);         /* smallest e with amax 2^-e <= 448 */
+void oracle_sgemm_fp8(char transA, char transB, int64_t M, int64_t N, int64_t K, double alpha,
+                      const float *A, int64_t lda, const float *B, int64_t ldb, double beta,
+                      const float *C32, int64_t ldc32, double *C64, int64_t ldc64);
+
 void oracle_set_threads(int n);
 int oracle_get_threads(void);
 

@@ -65,6 +65,14 @@ def lib() -> ctypes.CDLL:
         L.oracle_normwise_f64.argtypes = [_i64, _i64, _dp, _i64, _dp, _i64, ctypes.POINTER(ctypes.c_double)]
         L.oracle_normwise_f32.restype = ctypes.c_double
         L.oracle_normwise_f32.argtypes = [_i64, _i64, _dp, _i64, _dp, _i64, ctypes.POINTER(ctypes.c_double)]
+        L.oracle_fp8_e4m3.restype = ctypes.c_uint
+        L.oracle_fp8_e4m3.argtypes = [ctypes.c_float]
+        L.oracle_fp8_decode.restype = ctypes.c_float
+        L.oracle_fp8_decode.argtypes = [ctypes.c_uint]
+        L.oracle_fp8_block_exp.restype = ctypes.c_int
+        L.oracle_fp8_block_exp.argtypes = [ctypes.c_float]
+        L.oracle_sgemm_fp8.argtypes = [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, ctypes.c_double,
+                                       _dp, _i64, _dp, _i64, ctypes.c_double, _dp, _i64, _dp, _i64]
         L.oracle_set_threads.argtypes = [ctypes.c_int]
         L.oracle_get_threads.restype = ctypes.c_int
         _lib = L
@@ -158,6 +166,30 @@ def gemm_xp(transA: str, transB: str, A, B, C, alpha: float, beta: float) -> np.
         lib().oracle_gemm_xp_f32(transA.encode(), transB.encode(), M, N, K, alpha, _ptr(A), _ld(A),
                                  _ptr(B), _ld(B), beta, _ptr(C), _ld(C), _ptr(out), N)
     return out
+
+
+def gemm_fp8(transA: str, transB: str, A, B, C, alpha: float, beta: float) -> np.ndarray:
+    """The scaled-FP8 GEMM of float32 operands (quantized as the library
+    quantizes them, accumulated in extended precision); returns a new float64
+    M x N array."""
+    M, N = C.shape
+    K = A.shape[0] if transA == "T" else A.shape[1]
+    out = np.empty((M, N), dtype=np.float64)
+    lib().oracle_sgemm_fp8(transA.encode(), transB.encode(), M, N, K, alpha, _ptr(A), _ld(A),
+                           _ptr(B), _ld(B), beta, _ptr(C), _ld(C), _ptr(out), N)
+    return out
+
+
+def fp8_e4m3(x: float) -> int:
+    return int(lib().oracle_fp8_e4m3(x))
+
+
+def fp8_decode(q: int) -> float:
+    return float(lib().oracle_fp8_decode(q))
+
+
+def fp8_block_exp(amax: float) -> int:
+    return int(lib().oracle_fp8_block_exp(amax))
 
 
 def normwise(X, R) -> tuple[float, float]:

@@ -21,6 +21,22 @@ def test_library_exports_every_declared_function(blas):
         assert expected in names
 
 
+def test_binding_declares_every_function_with_arguments(blas):
+    """Every header function taking arguments has ctypes argument types in
+    the Python binding (an undeclared one would pass floats wrongly)."""
+    import re
+
+    L = blas.lib()
+    text = blas.HEADER.read_text()
+    undeclared = []
+    for name in blas.header_functions():
+        m = re.search(r"\b%s\s*\(([^)]*)\)" % re.escape(name), text)
+        takes_args = m is not None and m.group(1).strip() not in ("", "void")
+        if takes_args and getattr(L, name).argtypes is None:
+            undeclared.append(name)
+    assert not undeclared, undeclared
+
+
 def test_version_and_status_strings(blas):
     assert "gfx950" in blas.version()
     L = blas.lib()

@@ -274,3 +274,111 @@ def test_mode_invariance_with_padding(orc):
             if ref is None:
                 ref = R
             np.testing.assert_array_equal(R, ref)
+
+
+# ---- scaled FP8 restatement ------------------------------------------------
+
+def _fp8_sample(rng, n):
+    """Floats in [-448, 448]: random magnitudes over the whole E4M3 range, exact
+    ties between neighbouring codes, subnormals and the extremes."""
+    import torch
+
+    mags = np.exp2(rng.uniform(-12, np.log2(448.0), n)).astype(np.float32)
+    vals = mags * rng.choice([-1.0, 1.0], n).astype(np.float32)
+    codes = torch.arange(0, 127, dtype=torch.uint8).view(torch.float8_e4m3fn).float().numpy()
+    ties = ((codes[:-1] + codes[1:]) / 2).astype(np.float32)  # midpoints are exact in float32
+    extra = np.array([0.0, -0.0, 448.0, -448.0, 2.0 ** -9, 2.0 ** -10, 3 * 2.0 ** -10, 2.0 ** -6], dtype=np.float32)
+    return np.concatenate([vals, ties, -ties, codes, extra]).astype(np.float32)
+
+
+def test_fp8_encoder_matches_torch_e4m3fn(orc):
+    """Round-to-nearest-even E4M3 encoding, against PyTorch's independent
+    float8_e4m3fn conversion, including every tie between adjacent codes."""
+    import torch
+
+    x = _fp8_sample(np.random.default_rng(5), 20000)
+    want = torch.from_numpy(x).to(torch.float8_e4m3fn).view(torch.uint8).numpy()
+    got = np.array([orc.fp8_e4m3(float(v)) for v in x], dtype=np.uint8)
+    bad = np.nonzero(got != want)[0]
+    assert bad.size == 0, [(float(x[i]), int(got[i]), int(want[i])) for i in bad[:10]]
+
+
+def test_fp8_decode_every_code(orc):
+    import torch
+
+    codes = torch.arange(0, 256, dtype=torch.int32).to(torch.uint8)
+    want = codes.view(torch.float8_e4m3fn).float().numpy()
+    for c in range(256):
+        got = orc.fp8_decode(c)
+        if np.isnan(want[c]):
+            assert np.isnan(got), c
+        else:
+            assert got == want[c], (c, got, want[c])
+            if c not in (0x80,):  # -0 encodes back to -0 (0x80) as well
+                assert orc.fp8_e4m3(got) == c
+
+
+def test_fp8_block_exponent_is_the_smallest_fitting_power(orc):
+    rng = np.random.default_rng(6)
+    amax = np.exp2(rng.uniform(-100, 100, 2000)).astype(np.float32)
+    amax = np.concatenate([amax, np.float32([448.0, 448.0 * 2 ** 5, 449.0, 224.0, 1.0, 0.875 * 2 ** 9])])
+    for a in amax:
+        e = orc.fp8_block_exp(float(a))
+        assert float(a) * 2.0 ** -e <= 448.0, (a, e)
+        assert float(a) * 2.0 ** -(e - 1) > 448.0, (a, e)
+    assert orc.fp8_block_exp(0.0) == 0
+
+
+def test_fp8_gemm_integer_patterns_exact(orc):
+    """The %11 / %7 matrices are E4M3-exact after any power-of-two scaling
+    (integers below 16 have at most four significant bits): the quantized
+    product is the exact integer product, in every mode."""
+    M, N, K = 37, 29, 300
+    P = None
+    for ta in "NT":
+        for tb in "NT":
+            A = orc.matrix(M, K, orc.PATTERN11, 0, dtype=np.float32, trans=ta == "T")
+            B = orc.matrix(K, N, orc.PATTERN7, 0, dtype=np.float32, trans=tb == "T")
+            if P is None:
+                An = orc.matrix(M, K, orc.PATTERN11, 0, dtype=np.float32).astype(np.int64)
+                Bn = orc.matrix(K, N, orc.PATTERN7, 0, dtype=np.float32).astype(np.int64)
+                P = (An @ Bn).astype(np.float64)
+            C0 = np.zeros((M, N), dtype=np.float32)
+            got = orc.gemm_fp8(ta, tb, A, B, C0, 1.0, 0.0)
+            np.testing.assert_array_equal(got, P)
+
+
+def test_fp8_gemm_per_block_scales(orc):
+    """Two K blocks whose magnitudes differ by 2^40: each block keeps its own
+    scale, so values that are E4M3 codes times a power of two survive exactly
+    and the result equals the exact product (a single scale per row would
+    flush the small block to zero)."""
+    from fractions import Fraction
+
+    rng = np.random.default_rng(7)
+    M, N, K = 3, 4, 256
+    codes = np.array([1, 1.125, 1.25, 1.5, 1.75, 2, 3, 5, 7, 10, 14], dtype=np.float32)
+    A = rng.choice(codes, (M, K)).astype(np.float32) * rng.choice([-1, 1], (M, K)).astype(np.float32)
+    B = rng.choice(codes, (K, N)).astype(np.float32)
+    A[:, :128] *= np.float32(2.0 ** -20)
+    A[:, 128:] *= np.float32(2.0 ** 20)
+    C0 = np.zeros((M, N), dtype=np.float32)
+    got = orc.gemm_fp8("N", "N", A, B, C0, 1.0, 0.0)
+    for i in range(M):
+        for j in range(N):
+            exact = sum(Fraction(float(A[i, k])) * Fraction(float(B[k, j])) for k in range(K))
+            assert got[i, j] == float(exact), (i, j)
+
+
+def test_fp8_gemm_error_band(orc):
+    """Random normal operands: the quantized product's normwise error against
+    the exact product lies in the band E4M3 rounding implies (relative element
+    error below 2^-4, about 2^-4/sqrt(3) rms for each operand)."""
+    M, N, K = 64, 48, 1000
+    A = orc.matrix(M, K, orc.NORMAL, 8, dtype=np.float32)
+    B = orc.matrix(K, N, orc.NORMAL, 9, dtype=np.float32)
+    C0 = orc.matrix(M, N, orc.NORMAL, 10, dtype=np.float32)
+    ref = orc.gemm_xp("N", "N", A, B, C0, 1.0, 0.0)
+    got = orc.gemm_fp8("N", "N", A, B, C0, 1.0, 0.0)
+    err, _ = orc.normwise(got, ref)
+    assert 0.005 < err < 0.08, err
