@@ -349,7 +349,7 @@ def bench_copy(args):
     big = torch.ones(2 * n, dtype=torch.float32, device="cuda")
     sink = torch.empty(1 << 16, dtype=torch.int32, device="cuda")
     read_gbs = {}
-    for blocks in (1024, 2048, 4096, 8192, 16384):
+    for blocks in (4096, 16384, 32768, 65536):
         ms_r = _time_events(lambda: blas.check("blasdsp_probe_read", blas.lib().blasdsp_probe_read(
             big.data_ptr(), 8 * n, sink.data_ptr(), blocks, stream.cuda_stream)), max(3, min(args.steps, 20)), stream)
         read_gbs[blocks] = round(8 * n / ms_r / 1e6, 1)
