@@ -26,11 +26,12 @@
 //     (R = block / 4); element e of the vector is the B operand of column
 //     tile e, which therefore covers columns n0 + V*c + e (an interleaved
 //     column map undone at the store);
-//   * L stored nl-major ([nl][k], k contiguous): lane (c, g) loads the run
-//     k0 + g*R .. + R-1 of row n0+16t+c, one run per column tile t, so the
-//     four lane groups read one whole 128-byte line of each row.
-// In both cases virtual step j pairs lane group g with k = k0 + g*R + j, and
-// S is read with the same pairing (a run along k when S is k-contiguous).
+//   * L stored nl-major ([nl][k], k contiguous): lane (c, g) loads V
+//     consecutive k of row n0+16t+c per vector, the four lane groups side by
+//     side, so each load instruction reads 64 contiguous bytes of 16 rows
+//     and two of them the whole 128-byte line.
+// S is read with the same k pairing as L (k_of below; a run along k when S
+// is k-contiguous).
 // S is tiny (<= 64 x K) and is served from L1/L2.
 //
 // When the Nl panels alone do not give >= 4 workgroups per CU, K is split
@@ -111,8 +112,21 @@ __device__ __forceinline__ int col_of(int t, int c)
     return (t / V) * 16 * V + V * c + (t % V);
 }
 
-// Registers for one K block of one wave.  Virtual MFMA step j pairs lane
-// group g with k = k0 + g * RUN + j on both operands.
+// k paired with lane group g at virtual MFMA step j of the block at k0, the
+// same on both operands.  K-contiguous L: vector q of every lane group sits
+// side by side (k0 + 4 V q + V g + e), so one load instruction reads 64
+// contiguous bytes of each of its 16 rows.  k-major L: lane group g walks its
+// own run (k0 + RUN g + j), one row of L per step.
+template <typename T, bool L_KC>
+__device__ __forceinline__ int64_t k_of(int64_t k0, int g, int j)
+{
+    constexpr int V = Blk<T>::V, RUN = Blk<T>::RUN;
+    if (L_KC) return k0 + (j / V) * 4 * V + g * V + (j % V);
+    return k0 + g * RUN + j;
+}
+
+// Registers for one K block of one wave; step j pairs lane group g with
+// k_of(k0, g, j) on both operands.
 template <typename T, int MT, bool S_KC, bool L_KC>
 struct Block {
     using vec_t = typename SK<T>::vec_t;
@@ -155,17 +169,27 @@ __device__ __forceinline__ void load_block(Block<T, MT, S_KC, L_KC> &b, const Sk
 {
     using vec_t = typename SK<T>::vec_t;
     using B_ = Blk<T>;
-    constexpr int V = B_::V, RUN = B_::RUN;
+    constexpr int V = B_::V;
     // ---- L ----
-    if (L_KC) {
-        // column tile t: row n0 + 16t + c, k-run k0 + g*RUN .. + RUN-1 (a full
-        // 128-byte line per row across the four lane groups)
+    if (L_KC && !MASK) {
+        // staged: vector i of the lane is 16 bytes of row n0 + 8 i + lane / 8,
+        // chunk lane % 8, so each instruction reads whole 128-byte lines of 8
+        // rows; stage_block() turns these into MFMA operands through LDS.
+        const int lane = g * 16 + c;
+#pragma unroll
+        for (int i = 0; i < B_::NLV; ++i) {
+            const int64_t row = n0 + 8 * i + (lane >> 3);
+            b.l[i] = stream_load(reinterpret_cast<const vec_t *>(a.L + row * a.ldl + k0 + (lane & 7) * V));
+        }
+    } else if (L_KC) {
+        // column tile t: row n0 + 16t + c, vector q: k_of(k0, g, qV) .. + V-1
+        // (the four lane groups read 64 contiguous bytes of the row)
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
 #pragma unroll
             for (int q = 0; q < B_::RV; ++q) {
                 const int64_t row = n0 + 16 * t + c;
-                const int64_t kk = k0 + g * RUN + q * V;
+                const int64_t kk = k_of<T, true>(k0, g, q * V);
                 vec_t v;
                 if (!MASK) {
                     v = *reinterpret_cast<const vec_t *>(a.L + row * a.ldl + kk);
@@ -180,12 +204,12 @@ __device__ __forceinline__ void load_block(Block<T, MT, S_KC, L_KC> &b, const Sk
             }
         }
     } else {
-        // step j: row k0 + g*RUN + j, columns n0 + h*16V + V*c .. + V-1
+        // step j: row k_of(k0, g, j), columns n0 + h*16V + V*c .. + V-1
 #pragma unroll
         for (int j = 0; j < B_::NSTEP; ++j) {
 #pragma unroll
             for (int h = 0; h < 4 / V; ++h) {
-                const int64_t kk = k0 + g * RUN + j;
+                const int64_t kk = k_of<T, false>(k0, g, j);
                 const int64_t col = n0 + h * 16 * V + V * c;
                 vec_t v;
                 if (!MASK) {
@@ -208,7 +232,7 @@ __device__ __forceinline__ void load_block(Block<T, MT, S_KC, L_KC> &b, const Sk
         if (S_KC) {
 #pragma unroll
             for (int q = 0; q < B_::RV; ++q) {
-                const int64_t kk = k0 + g * RUN + q * V;
+                const int64_t kk = k_of<T, L_KC>(k0, g, q * V);
                 vec_t v;
                 if (ms < a.Ms && a.s_vec && (!MASK || kk + V <= kend)) {
                     v = *reinterpret_cast<const vec_t *>(a.S + ms * a.lds + kk);
@@ -222,11 +246,43 @@ __device__ __forceinline__ void load_block(Block<T, MT, S_KC, L_KC> &b, const Sk
         } else {
 #pragma unroll
             for (int j = 0; j < B_::NSTEP; ++j) {
-                const int64_t kk = k0 + g * RUN + j;
+                const int64_t kk = k_of<T, L_KC>(k0, g, j);
                 b.s_scl[mt][j] = (ms < a.Ms && (!MASK || kk < kend)) ? a.S[kk * a.lds + ms] : T(0);
             }
         }
     }
+}
+
+// LDS row stride of the staging buffer: one 128-byte K block plus 16 bytes,
+// so the 16-byte operand reads of 8 consecutive lanes hit distinct banks.
+template <typename T>
+struct Stage {
+    static constexpr int ROW = Blk<T>::KB + 16 / (int)sizeof(T);
+    static constexpr int ELEMS = kPanel * ROW;   // per wave
+};
+
+// Staged K-contiguous L: write the lane's raw line pieces to the wave's own
+// LDS rows, then read back the MFMA operands (row 16 t + c, k_of(k0, g, qV)
+// .. + V-1).  Only this wave touches its rows and LDS operations of one wave
+// execute in order, so no barrier is needed; the wave barrier keeps the
+// compiler from moving the reads above the writes.
+template <typename T, int MT, bool S_KC>
+__device__ __forceinline__ void stage_block(Block<T, MT, S_KC, true> &b, T *w, int c, int g)
+{
+    using vec_t = typename SK<T>::vec_t;
+    using B_ = Blk<T>;
+    constexpr int V = B_::V, ROW = Stage<T>::ROW;
+    const int lane = g * 16 + c;
+#pragma unroll
+    for (int i = 0; i < B_::NLV; ++i)
+        *reinterpret_cast<vec_t *>(w + (8 * i + (lane >> 3)) * ROW + (lane & 7) * V) = b.l[i];
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int q = 0; q < B_::RV; ++q)
+            b.l[t * B_::RV + q] = *reinterpret_cast<const vec_t *>(w + (16 * t + c) * ROW + q * 4 * V + g * V);
+    __builtin_amdgcn_wave_barrier();
 }
 
 template <typename T, int MT, bool S_KC, bool L_KC>
@@ -272,6 +328,10 @@ skinny_kernel(SkinnyArgs<T> a)
     const bool panel_full = a.l_vec && (n0 + kPanel <= a.Nl);
     const int64_t kfull = panel_full ? kb + ((ke - kb) / KB) * KB : kb;
 
+    // K-contiguous L is staged through the wave's own LDS rows.
+    __shared__ __attribute__((aligned(16))) T stage[L_KC ? kWaves * Stage<T>::ELEMS : 1];
+    T *const w = stage + (L_KC ? wave * Stage<T>::ELEMS : 0);
+
     // Full blocks: unmasked L loads, register double buffer.
     int64_t k0 = kb;
     if (k0 < kfull) {
@@ -279,9 +339,11 @@ skinny_kernel(SkinnyArgs<T> a)
         load_block<T, MT, S_KC, L_KC, false>(cur, a, n0, k0, ke, c, g);
         for (; k0 + KB < kfull; k0 += KB) {
             load_block<T, MT, S_KC, L_KC, false>(nxt, a, n0, k0 + KB, ke, c, g);
+            if constexpr (L_KC) stage_block<T, MT, S_KC>(cur, w, c, g);
             mma_block<T, MT, S_KC, L_KC>(acc, cur);
             cur = nxt;
         }
+        if constexpr (L_KC) stage_block<T, MT, S_KC>(cur, w, c, g);
         mma_block<T, MT, S_KC, L_KC>(acc, cur);
         k0 = kfull;
     }
