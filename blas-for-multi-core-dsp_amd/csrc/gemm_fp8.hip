@@ -74,49 +74,87 @@ __device__ __forceinline__ int64_t scale_byte(int64_t r, int64_t kb, int64_t Rp)
     return ((kb * (Rp / kRowBlock) + r / kRowBlock) * 16 + (r % 16)) * 4 + (r % kRowBlock) / 16;
 }
 
-// Rows of op(X) stored K-contiguously: element (r, k) at X[r ld + k].  One
-// wave per (row, k block), two elements per lane.
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// Four E4M3 codes of x 2^-e packed little-endian.
+__device__ __forceinline__ uint32_t pack4(f4v x, int e)
+{
+    return e4m3(ldexpf(x[0], -e)) | (e4m3(ldexpf(x[1], -e)) << 8) | (e4m3(ldexpf(x[2], -e)) << 16) |
+           (e4m3(ldexpf(x[3], -e)) << 24);
+}
+
+// Rows of op(X) stored K-contiguously: element (r, k) at X[r ld + k].  A
+// half wave takes one (row, k block), four consecutive k per lane; each wave
+// walks kQuantBlocks pairs of blocks.  VEC: 16-byte loads (X 16-byte aligned
+// and ld a multiple of 4).
+constexpr int kQuantBlocks = 4;
+
+template <bool VEC>
 __global__ void __launch_bounds__(256) quant_kc_kernel(const float *__restrict__ X, int64_t ld, int64_t R,
                                                        int64_t K, int64_t Rp, int64_t KB,
                                                        uint8_t *__restrict__ Q, uint8_t *__restrict__ S)
 {
-    const int64_t id = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (id >= Rp * KB) return;  // whole waves only; no barrier below
-    const int lane = threadIdx.x & 63;
-    const int64_t r = id / KB, kb = id % KB, k = kb * kQB + 2 * lane;
-    float x0 = 0.0f, x1 = 0.0f;
-    if (r < R) {
-        if (k < K) x0 = X[r * ld + k];
-        if (k + 1 < K) x1 = X[r * ld + k + 1];
-    }
-    float amax = fmaxf(fabsf(x0), fabsf(x1));
+    const int lane = threadIdx.x & 63, half = lane >> 5, hl = lane & 31;
+    const int64_t wave_id = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t nblk = Rp * KB;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) amax = fmaxf(amax, __shfl_xor(amax, o));
-    const int e = block_exp(amax);
-    const uint32_t q = e4m3(ldexpf(x0, -e)) | (e4m3(ldexpf(x1, -e)) << 8);
-    *reinterpret_cast<uint16_t *>(Q + r * (KB * kQB) + k) = (uint16_t)q;
-    if (lane == 0) S[scale_byte(r, kb, Rp)] = (uint8_t)(127 + e);
+    for (int it = 0; it < kQuantBlocks; ++it) {
+        const int64_t id = (wave_id * kQuantBlocks + it) * 2 + half;
+        if (id >= nblk) break;  // uniform per half wave; no cross-half exchange below
+        const int64_t r = id / KB, kb = id % KB, k = kb * kQB + 4 * hl;
+        f4v x = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (r < R) {
+            if (VEC && k + 3 < K) {
+                x = __builtin_nontemporal_load(reinterpret_cast<const f4v *>(X + r * ld + k));
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (k + e < K) x[e] = X[r * ld + k + e];
+            }
+        }
+        float amax = fmaxf(fmaxf(fabsf(x[0]), fabsf(x[1])), fmaxf(fabsf(x[2]), fabsf(x[3])));
+#pragma unroll
+        for (int o = 16; o > 0; o >>= 1) amax = fmaxf(amax, __shfl_xor(amax, o));
+        const int e = block_exp(amax);
+        *reinterpret_cast<uint32_t *>(Q + r * (KB * kQB) + k) = pack4(x, e);
+        if (hl == 0) S[scale_byte(r, kb, Rp)] = (uint8_t)(127 + e);
+    }
 }
 
 // Rows of op(X) stored transposed: element (r, k) at X[k ld + r].  A
-// workgroup takes 64 rows x one k block through LDS: coalesced loads along
-// r, then each thread quantizes 32 consecutive k of one row.
+// workgroup takes 64 rows x one k block through LDS.  Loads: thread
+// (r4, kq) = (t % 16, t / 16) reads rows 4 r4 .. + 3 of k rows kq + 16 i as
+// one 16-byte vector (VEC: X 16-byte aligned, ld a multiple of 4); then each
+// thread quantizes 32 consecutive k of one row.
+template <bool VEC>
 __global__ void __launch_bounds__(256) quant_mc_kernel(const float *__restrict__ X, int64_t ld, int64_t R,
                                                        int64_t K, int64_t Rp, int64_t KB,
                                                        uint8_t *__restrict__ Q, uint8_t *__restrict__ S)
 {
     __shared__ float T[64][kQB + 1];
     __shared__ float wmax[4][64];
-    const int tid = threadIdx.x, rl = tid & 63, w = tid >> 6;
-    const int64_t r = (int64_t)blockIdx.x * 64 + rl, kb = blockIdx.y, k0 = kb * kQB;
-#pragma unroll 8
-    for (int i = 0; i < 32; ++i) {
-        const int k = w + 4 * i;
-        float v = 0.0f;
-        if (r < R && k0 + k < K) v = X[(k0 + k) * ld + r];
-        T[rl][k] = v;
+    const int tid = threadIdx.x, r4 = tid & 15, kq = tid >> 4;
+    const int64_t r0 = (int64_t)blockIdx.x * 64, kb = blockIdx.y, k0 = kb * kQB;
+    const int64_t rv = r0 + 4 * r4;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int k = kq + 16 * i;
+        f4v v = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (k0 + k < K) {
+            if (VEC && rv + 3 < R) {
+                v = __builtin_nontemporal_load(reinterpret_cast<const f4v *>(X + (k0 + k) * ld + rv));
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (rv + e < R) v[e] = X[(k0 + k) * ld + rv + e];
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) T[4 * r4 + e][k] = v[e];
     }
     __syncthreads();
+    const int rl = tid & 63, w = tid >> 6;
+    const int64_t r = r0 + rl;
     float amax = 0.0f;
 #pragma unroll
     for (int j = 0; j < 32; ++j) amax = fmaxf(amax, fabsf(T[rl][w * 32 + j]));
@@ -127,10 +165,9 @@ __global__ void __launch_bounds__(256) quant_mc_kernel(const float *__restrict__
     u32x4 out[2];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-        uint32_t word = 0;
-#pragma unroll
-        for (int b = 0; b < 4; ++b) word |= e4m3(ldexpf(T[rl][w * 32 + 4 * j + b], -e)) << (8 * b);
-        out[j / 4][j % 4] = word;
+        const f4v x = {T[rl][w * 32 + 4 * j], T[rl][w * 32 + 4 * j + 1], T[rl][w * 32 + 4 * j + 2],
+                       T[rl][w * 32 + 4 * j + 3]};
+        out[j / 4][j % 4] = pack4(x, e);
     }
     u32x4 *dst = reinterpret_cast<u32x4 *>(Q + r * (KB * kQB) + k0 + w * 32);
     dst[0] = out[0];
@@ -279,13 +316,16 @@ Fp8Layout fp8_layout(int64_t M, int64_t N, int64_t K, void *base)
 hipError_t quantize(const float *X, int64_t ld, bool k_contig, int64_t R, int64_t K, int64_t Rp, int64_t KB,
                     uint8_t *Q, uint8_t *S, hipStream_t s)
 {
+    const bool vec = ((uintptr_t)X % 16) == 0 && ld % 4 == 0;
     if (k_contig) {
-        const int64_t waves = Rp * KB;
-        hipLaunchKernelGGL(quant_kc_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, X, ld, R, K, Rp,
-                           KB, Q, S);
+        const int64_t per_wave = 2 * kQuantBlocks, waves = (Rp * KB + per_wave - 1) / per_wave;
+        const dim3 grid((unsigned)((waves + 3) / 4));
+        if (vec) hipLaunchKernelGGL(quant_kc_kernel<true>, grid, dim3(256), 0, s, X, ld, R, K, Rp, KB, Q, S);
+        else hipLaunchKernelGGL(quant_kc_kernel<false>, grid, dim3(256), 0, s, X, ld, R, K, Rp, KB, Q, S);
     } else {
-        hipLaunchKernelGGL(quant_mc_kernel, dim3((unsigned)(Rp / 64), (unsigned)KB), dim3(256), 0, s, X, ld, R, K,
-                           Rp, KB, Q, S);
+        const dim3 grid((unsigned)(Rp / 64), (unsigned)KB);
+        if (vec) hipLaunchKernelGGL(quant_mc_kernel<true>, grid, dim3(256), 0, s, X, ld, R, K, Rp, KB, Q, S);
+        else hipLaunchKernelGGL(quant_mc_kernel<false>, grid, dim3(256), 0, s, X, ld, R, K, Rp, KB, Q, S);
     }
     return hipGetLastError();
 }
