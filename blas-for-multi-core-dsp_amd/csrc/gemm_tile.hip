@@ -10,9 +10,9 @@
 // sgemm_release_V1.3.2/device_code/sgemm_kernel.c:2476-4500).  The per-tile
 // work is tile::tile_body (tile_body.h); this file launches one workgroup per
 // C tile.  In summary:
-//   * one 256-thread workgroup owns a 128 x 128 tile of C (64 x 64 when the
-//     128 tiles would not cover the CUs); its four waves own quarter tiles of
-//     4 x 4 (or 2 x 2) MFMA 16x16 accumulators each;
+//   * one workgroup owns a 128 x 128 tile of C (64 x 64 when the 128 tiles
+//     would not cover the CUs): 8 waves of 32 x 64 for the FP64 128 tile,
+//     4 waves of quarter tiles otherwise;
 //   * op(A) / op(B) tiles of BK = 128 bytes of K are staged global -> LDS with
 //     16-byte vector loads along whichever dimension is contiguous in HBM, and
 //     the LDS copy keeps that same orientation.  The MFMA fragment reads adapt
@@ -52,14 +52,28 @@ __device__ __forceinline__ void tile_of_block(int tiles_m, int tiles_n, int &tm,
     tn = (pid % width) / gsz;
 }
 
+// Waves per workgroup of the per-call kernel: 8 for the FP64 128 tile (each
+// wave 32 x 64, four waves per SIMD), 4 elsewhere.
+template <typename T, int BM>
+struct PerCall {
+    static constexpr int NW = (sizeof(T) == 8 && BM == 128) ? 8 : 4;
+    using Cf = Cfg<T, BM, NW>;
+};
+
 template <typename T, int BM, bool TA, bool TB>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Cfg<T, BM>::WPE, Cfg<T, BM>::WPE)))
+__global__ void __launch_bounds__((PerCall<T, BM>::Cf::THREADS))
+__attribute__((amdgpu_waves_per_eu(PerCall<T, BM>::Cf::WPE, PerCall<T, BM>::Cf::WPE)))
 tile_gemm_kernel(Gemm<T> g, int tiles_m, int tiles_n, bool a_vec, bool b_vec)
 {
-    __shared__ __attribute__((aligned(16))) T lds[Cfg<T, BM>::LDS_ELEMS];
+    using Cf = typename PerCall<T, BM>::Cf;
+    __shared__ __attribute__((aligned(16))) T lds[Cf::LDS_ELEMS];
     int tm, tn;
     tile_of_block(tiles_m, tiles_n, tm, tn);
-    tile::tile_body<T, BM, TA, TB>(g, (int64_t)tm * BM, (int64_t)tn * BM, a_vec, b_vec, lds);
+    const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BM;
+    if (tile::full_tile<Cf>(g, m0, n0, a_vec, b_vec))
+        tile::tile_body<T, BM, TA, TB, Cf::NW, true>(g, m0, n0, a_vec, b_vec, lds);
+    else
+        tile::tile_body<T, BM, TA, TB, Cf::NW, false>(g, m0, n0, a_vec, b_vec, lds);
 }
 
 template <typename T>
@@ -88,7 +102,7 @@ hipError_t launch_tile_bm(const Gemm<T> &g, hipStream_t s)
     if (nwg > 0x7fffffffLL) return hipErrorInvalidValue;
     const bool a_vec = tile::vec_ok(g.A, g.lda);
     const bool b_vec = tile::vec_ok(g.B, g.ldb);
-    const dim3 grid((unsigned)nwg), block(256);
+    const dim3 grid((unsigned)nwg), block(PerCall<T, BM>::Cf::THREADS);
     const int tmi = (int)tiles_m, tni = (int)tiles_n;
     if (!g.ta && !g.tb)
         hipLaunchKernelGGL((tile_gemm_kernel<T, BM, false, false>), grid, block, 0, s, g, tmi, tni, a_vec, b_vec);
