@@ -93,7 +93,11 @@ __device__ __forceinline__ void store_scale(float amax, int64_t r, float *scale,
     inv[r] = ldexpf(1.0f, -e);
 }
 
-// X stored [R][K]: one workgroup per row (rows R .. Rp-1 get scale 1).
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// X stored [R][K]: one workgroup per row (rows R .. Rp-1 get scale 1).  VEC:
+// 16-byte loads (X 16-byte aligned, ld a multiple of 4).
+template <bool VEC>
 __global__ void __launch_bounds__(256) rowscale_rows_kernel(const float *__restrict__ X, int64_t ld, int64_t R,
                                                             int64_t K, float *__restrict__ scale,
                                                             float *__restrict__ inv)
@@ -101,8 +105,20 @@ __global__ void __launch_bounds__(256) rowscale_rows_kernel(const float *__restr
     __shared__ float part[256];
     const int64_t r = blockIdx.x;
     float amax = 0.0f;
-    if (r < R)
-        for (int64_t k = threadIdx.x; k < K; k += 256) amax = fmaxf(amax, fabsf(X[r * ld + k]));
+    if (r < R) {
+        const float *row = X + r * ld;
+        int64_t k = 0;
+        if (VEC) {
+            for (k = 4 * (int64_t)threadIdx.x; k + 3 < K; k += 1024) {
+                const f4v v = __builtin_nontemporal_load(reinterpret_cast<const f4v *>(row + k));
+                amax = fmaxf(amax, fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fmaxf(fabsf(v[2]), fabsf(v[3]))));
+            }
+            k = K / 4 * 4 + threadIdx.x;  // the ragged end, one element per thread
+            if (k < K) amax = fmaxf(amax, fabsf(row[k]));
+        } else {
+            for (k = threadIdx.x; k < K; k += 256) amax = fmaxf(amax, fabsf(row[k]));
+        }
+    }
     part[threadIdx.x] = amax;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -137,54 +153,78 @@ __global__ void __launch_bounds__(256) colscale_kernel(const float *__restrict__
     store_scale(amax, r, scale, inv);
 }
 
-// One 64 x 64 tile of op(X) (R logical rows x K) into NP planes [Rp][Kp].
-// KC: X is stored [R][K] (k contiguous); else [K][R].
-template <int NP, bool FP16, bool KC>
+// Pieces of four consecutive k of one row, written as one 8-byte store per
+// plane (planes are [Rp][Kp] with Kp a multiple of 64: 8-byte aligned).
+template <int NP, bool FP16>
+__device__ __forceinline__ void split_store4(f4v x, float sc, uint16_t *__restrict__ planes, int64_t plane_elems,
+                                             int64_t off)
+{
+    uint32_t p[4][NP];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) split<NP, FP16>(x[j], sc, p[j]);
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+        const uint2 w = make_uint2(p[0][q] | (p[1][q] << 16), p[2][q] | (p[3][q] << 16));
+        *reinterpret_cast<uint2 *>(planes + q * plane_elems + off) = w;
+    }
+}
+
+// One 64 x 64 tile of op(X) (R logical rows x K) into NP planes [Rp][Kp]:
+// thread (row, k quad) = (e / 16, e % 16) for e = tid + 256 i.  KC: X is
+// stored [R][K] (k contiguous), read as 16-byte vectors along k; else [K][R],
+// read as 16-byte vectors along r into an LDS tile and taken back along k.
+// VEC: 16-byte loads allowed (X 16-byte aligned, ld a multiple of 4).
+template <int NP, bool FP16, bool KC, bool VEC>
 __global__ void __launch_bounds__(256) split_kernel(const float *__restrict__ X, int64_t ld, int64_t R,
                                                    int64_t K, uint16_t *__restrict__ planes,
                                                    int64_t Kp, int64_t plane_elems,
                                                    const float *__restrict__ scale)
 {
-    __shared__ float tile[64][65];
+    __shared__ float tile[64][65];  // [r][k]
     const int tid = threadIdx.x;
     const int64_t k0 = (int64_t)blockIdx.x * 64, r0 = (int64_t)blockIdx.y * 64;
-    if (KC) {
+    if (!KC) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int e = tid + i * 256;  // 1024 groups of 4
-            const int row = e / 16, kq = (e % 16) * 4;
-            const int64_t gr = r0 + row;
-            const float sc = FP16 ? scale[gr] : 1.0f;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int64_t gk = k0 + kq + j;
-                const float x = (gr < R && gk < K) ? X[gr * ld + gk] : 0.0f;
-                uint32_t p[NP];
-                split<NP, FP16>(x, sc, p);
-#pragma unroll
-                for (int q = 0; q < NP; ++q) planes[q * plane_elems + gr * Kp + gk] = (uint16_t)p[q];
-            }
-        }
-    } else {
-        // coalesced reads along r into LDS, then writes along k
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
             const int e = tid + i * 256;
-            const int kk = e / 64, rr = e % 64;
-            const int64_t gk = k0 + kk, gr = r0 + rr;
-            tile[kk][rr] = (gr < R && gk < K) ? X[gk * ld + gr] : 0.0f;
+            const int kk = e / 16, rq = (e % 16) * 4;
+            const int64_t gk = k0 + kk, gr = r0 + rq;
+            f4v v = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (gk < K) {
+                if (VEC && gr + 3 < R) {
+                    v = __builtin_nontemporal_load(reinterpret_cast<const f4v *>(X + gk * ld + gr));
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        if (gr + c < R) v[c] = X[gk * ld + gr + c];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) tile[rq + c][kk] = v[c];
         }
         __syncthreads();
+    }
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int e = tid + i * 256;
-            const int rr = e / 64, kk = e % 64;
-            const int64_t gr = r0 + rr, gk = k0 + kk;
-            uint32_t p[NP];
-            split<NP, FP16>(tile[kk][rr], FP16 ? scale[gr] : 1.0f, p);
+    for (int i = 0; i < 4; ++i) {
+        const int e = tid + i * 256;
+        const int row = e / 16, kq = (e % 16) * 4;
+        const int64_t gr = r0 + row, gk = k0 + kq;
+        f4v x = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (KC) {
+            if (gr < R) {
+                if (VEC && gk + 3 < K) {
+                    x = __builtin_nontemporal_load(reinterpret_cast<const f4v *>(X + gr * ld + gk));
+                } else {
 #pragma unroll
-            for (int q = 0; q < NP; ++q) planes[q * plane_elems + gr * Kp + gk] = (uint16_t)p[q];
+                    for (int j = 0; j < 4; ++j)
+                        if (gk + j < K) x[j] = X[gr * ld + gk + j];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) x[j] = tile[row][kq + j];
         }
+        split_store4<NP, FP16>(x, FP16 ? scale[gr] : 1.0f, planes, plane_elems, gr * Kp + gk);
     }
 }
 
@@ -367,12 +407,16 @@ hipError_t launch_split(const float *X, int64_t ld, bool kc, int64_t R, int64_t 
                         int64_t Rp, int64_t Kp, const float *scale, hipStream_t s)
 {
     const dim3 grid((unsigned)(Kp / 64), (unsigned)(Rp / 64));
-    if (kc)
-        hipLaunchKernelGGL((split_kernel<NP, FP16, true>), grid, dim3(256), 0, s, X, ld, R, K, planes, Kp, Rp * Kp,
-                           scale);
+    const bool vec = ((uintptr_t)X % 16) == 0 && ld % 4 == 0;
+    const int64_t pe = Rp * Kp;
+    if (kc && vec)
+        hipLaunchKernelGGL((split_kernel<NP, FP16, true, true>), grid, dim3(256), 0, s, X, ld, R, K, planes, Kp, pe, scale);
+    else if (kc)
+        hipLaunchKernelGGL((split_kernel<NP, FP16, true, false>), grid, dim3(256), 0, s, X, ld, R, K, planes, Kp, pe, scale);
+    else if (vec)
+        hipLaunchKernelGGL((split_kernel<NP, FP16, false, true>), grid, dim3(256), 0, s, X, ld, R, K, planes, Kp, pe, scale);
     else
-        hipLaunchKernelGGL((split_kernel<NP, FP16, false>), grid, dim3(256), 0, s, X, ld, R, K, planes, Kp, Rp * Kp,
-                           scale);
+        hipLaunchKernelGGL((split_kernel<NP, FP16, false, false>), grid, dim3(256), 0, s, X, ld, R, K, planes, Kp, pe, scale);
     return hipGetLastError();
 }
 
@@ -382,7 +426,10 @@ hipError_t launch_rowscale(const float *X, int64_t ld, bool kc, int64_t R, int64
                            int64_t Rp, float *part, hipStream_t s)
 {
     if (kc) {
-        hipLaunchKernelGGL(rowscale_rows_kernel, dim3((unsigned)Rp), dim3(256), 0, s, X, ld, R, K, scale, inv);
+        if (((uintptr_t)X % 16) == 0 && ld % 4 == 0)
+            hipLaunchKernelGGL(rowscale_rows_kernel<true>, dim3((unsigned)Rp), dim3(256), 0, s, X, ld, R, K, scale, inv);
+        else
+            hipLaunchKernelGGL(rowscale_rows_kernel<false>, dim3((unsigned)Rp), dim3(256), 0, s, X, ld, R, K, scale, inv);
         return hipGetLastError();
     }
     const int64_t kslice = (K + kScaleSlices - 1) / kScaleSlices;

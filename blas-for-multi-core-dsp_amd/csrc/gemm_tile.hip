@@ -56,7 +56,7 @@ __device__ __forceinline__ void tile_of_block(int tiles_m, int tiles_n, int &tm,
 // wave 32 x 64, four waves per SIMD), 4 elsewhere.
 template <typename T, int BM>
 struct PerCall {
-    static constexpr int NW = (sizeof(T) == 8 && BM == 128) ? 8 : 4;
+    static constexpr int NW = BM == 128 ? 8 : 4;
     using Cf = Cfg<T, BM, NW>;
 };
 
