@@ -354,18 +354,29 @@ planes_kernel(const uint16_t *__restrict__ Ap, const uint16_t *__restrict__ Bp, 
             for (int j = 0; j < 4; ++j)
                 fb[p][j] = *reinterpret_cast<const u32x4 *>(&lds[cur][1][p][(wn + j * 16 + lr) * kLdsStride + lg * 8]);
         }
+        // Every accumulator sees its terms in the order t = 0, 1, ... in both
+        // loop orders.  One accumulator per tile (two-level forms): terms
+        // outermost, so consecutive MFMAs never share an accumulator.  Two
+        // accumulators (FP16 form): tiles outermost, the three terms of a
+        // tile back to back (measured 10 % faster at 8192^3).
+        if (Sc::CHUNK) {
 #pragma unroll
-        for (int t = 0; t < Sc::NT; ++t)
+            for (int t = 0; t < Sc::NT; ++t)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        part[Sc::CHUNK ? i : 0][Sc::CHUNK ? j : 0] = mfma16<Sc::FP16>(
+                            fa[Sc::a[t]][i], fb[Sc::b[t]][j], part[Sc::CHUNK ? i : 0][Sc::CHUNK ? j : 0]);
+        } else {
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    if (Sc::CHUNK)
-                        part[Sc::CHUNK ? i : 0][Sc::CHUNK ? j : 0] =
-                            mfma16<Sc::FP16>(fa[Sc::a[t]][i], fb[Sc::b[t]][j], part[Sc::CHUNK ? i : 0][Sc::CHUNK ? j : 0]);
-                    else
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int t = 0; t < Sc::NT; ++t)
                         acc[Sc::acc[t]][i][j] = mfma16<Sc::FP16>(fa[Sc::a[t]][i], fb[Sc::b[t]][j], acc[Sc::acc[t]][i][j]);
-                }
+        }
         if (Sc::CHUNK && ((kt + 1) % kChunkTiles == 0 || !more)) {
 #pragma unroll
             for (int i = 0; i < 4; ++i)
