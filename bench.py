@@ -107,14 +107,15 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--workload", default="dgemm", choices=["dgemm", "skinny", "precision", "serving", "copy", "grouped"])
+    ap.add_argument("--workload", default="dgemm", choices=["dgemm", "skinny", "precision", "serving", "copy", "grouped",
+                                                        "transpose"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1"}))
         sys.exit(2)
     if args.workload != "dgemm":
         {"skinny": bench_skinny, "precision": bench_precision, "serving": bench_serving,
-         "copy": bench_copy, "grouped": bench_grouped}[args.workload](args)
+         "copy": bench_copy, "grouped": bench_grouped, "transpose": bench_transpose}[args.workload](args)
         return
 
     import torch
@@ -414,6 +415,39 @@ def serving_requests(blas, n=10000, seed=4, rate=None):
             t += rng.exponential(1.0 / rate)
         reqs.append(blas.Request(dt, mode[0].encode(), mode[1].encode(), M, N, K, 1.0, 1.0, 4_000_000 + i, t))
     return reqs
+
+
+def bench_transpose(args):
+    """B = A^T, FP32 and FP64: 16384 x 16384 and the reference harness's 8210 x 8190
+    (transpose/bin/run.sh).  Algorithmic bytes: every element read and written once."""
+    import torch
+    import __graft_entry__
+
+    blas = __graft_entry__.load_package()
+    stream = torch.cuda.current_stream()
+    reps = max(3, min(args.steps, 20))
+    cases = []
+    tot_bytes = tot_ms = 0.0
+    for dt, s in ((torch.float32, 4), (torch.float64, 8)):
+        for M, N in ((16384, 16384), (8210, 8190)):
+            A = torch.empty((M, N), dtype=dt, device="cuda")
+            blas.fill(A, M, N, blas.UNIFORM, 6)
+            B = torch.empty((N, M), dtype=dt, device="cuda")
+            ms = _time_events(lambda: blas.transpose(A, B), reps, stream)
+            nbytes = 2.0 * M * N * s
+            cases.append({"dtype": "f32" if s == 4 else "f64", "M": M, "N": N, "ms": round(ms, 4),
+                          "gbs": round(nbytes / ms / 1e6, 1), "g_elements_per_s": round(M * N / ms / 1e6, 2)})
+            tot_bytes += nbytes
+            tot_ms += ms
+            del A, B
+    gbs = tot_bytes / tot_ms / 1e6
+    out = _common(args, round(gbs, 1), "GB/s", "f32+f64",
+                  {"workload": "out-of-place transpose B = A^T, 16384^2 and 8210 x 8190, FP32 and FP64",
+                   "cases": cases},
+                  {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                   "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None},
+                  ms_per_step=round(tot_ms / len(cases), 4))
+    print(json.dumps(out), flush=True)
 
 
 def bench_serving(args):

@@ -141,6 +141,12 @@ def _declare(L: ctypes.CDLL) -> None:
     L.blasdsp_tune_load.argtypes = [ctypes.c_char_p]
     L.blasdsp_tune_clear.argtypes = []
     L.blasdsp_probe_read.argtypes = [_vp, _i64, _vp, ctypes.c_int, _vp]
+    for name in ("transpose_16x16_dbuf", "transpose_float_32x32_dbuf", "transpose_null", "transpose_float_null",
+                 "transpose_A2B_float32"):
+        getattr(L, name).argtypes = [_i64, _i64, _vp, _vp, _vp]
+    L.transpose_A2B.argtypes = [_i64, _i64, _i64, _i64, _vp, _vp, _vp]
+    L.blasdsp_transpose_d.argtypes = [_i64, _i64, _vp, _i64, _vp, _i64, _vp]
+    L.blasdsp_transpose_s.argtypes = [_i64, _i64, _vp, _i64, _vp, _i64, _vp]
     for extra in _EXTRA_DECLS:
         extra(L)
 
@@ -320,6 +326,17 @@ def sgemm_fp8(transA: str, transB: str, A, B, C, alpha: float = 1.0, beta: float
     check("blasdsp_sgemm_fp8",
           lib().blasdsp_sgemm_fp8(transA.encode(), transB.encode(), M, N, K, alpha, A.data_ptr(), _ld(A),
                                   B.data_ptr(), _ld(B), beta, C.data_ptr(), _ld(C), _stream(stream)))
+
+
+def transpose(A, B, stream=None) -> None:
+    """B = A^T on torch device tensors (float32 or float64; 2-D, unit column stride)."""
+    import torch
+
+    M, N = A.shape
+    if B.shape != (N, M) or A.dtype != B.dtype:
+        raise ValueError("B must be A.shape[::-1] with A's dtype")
+    fn = lib().blasdsp_transpose_d if A.dtype == torch.float64 else lib().blasdsp_transpose_s
+    check("blasdsp_transpose", fn(M, N, A.data_ptr(), _ld(A), B.data_ptr(), _ld(B), _stream(stream)))
 
 
 def fill(X, rows: int, cols: int, dist: int, seed: int, trans: bool = False, pad: float = 0.0, stream=None):

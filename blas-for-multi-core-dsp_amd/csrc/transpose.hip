@@ -1,0 +1,156 @@
+// transpose.hip — out-of-place matrix transpose B = A^T (FP64 and FP32).
+//
+// What it replaces.  The reference's stand-alone transpose operator
+// (transpose/device_code/kernel.c:1167 transpose_16x16_dbuf for FP64, :2002
+// transpose_float_32x32_dbuf for FP32, launched by transpose/host_code/
+// transpose.c:25-166) and the exported transpose_A2B / transpose_A2B_float32
+// (dgemm_release_V1.1/include/dgemm_kernel.h:8, sgemm_release_V1.3.2/include/
+// sgemm_kernel.h:8).  There each DSP core DMAs 128 x 128 blocks into its
+// vector memory, transposes them with 16 x 16 (or 32 x 32) register
+// micro-kernels and DMAs them back.
+//
+// Here one workgroup moves a 64 x 64 tile: 16-byte loads along the rows of
+// A, scalar writes into a padded LDS tile at the transposed position, then
+// 16-byte stores along the rows of B.  Every element crosses HBM once each
+// way; the kernel is HBM-bound (algorithmic bytes = 2 * M * N * sizeof(T)).
+// Edge tiles and unaligned operands take element-wise accesses.
+#include "api_common.h"
+
+namespace blasdsp {
+namespace {
+
+constexpr int kTT = 64;  // tile edge
+
+template <typename T>
+struct TVec;
+template <>
+struct TVec<float> {
+    typedef float v __attribute__((ext_vector_type(4)));
+    static constexpr int V = 4;
+};
+template <>
+struct TVec<double> {
+    typedef double v __attribute__((ext_vector_type(2)));
+    static constexpr int V = 2;
+};
+
+// A: M x N (lda), B: N x M (ldb).  Tile (by, bx) covers rows by*64.. of A and
+// columns bx*64..  VEC: both operands 16-byte aligned with ld a multiple of V.
+template <typename T, bool VEC>
+__global__ void __launch_bounds__(256) transpose_kernel(const T *__restrict__ A, int64_t lda, T *__restrict__ B,
+                                                        int64_t ldb, int64_t M, int64_t N)
+{
+    using vec = typename TVec<T>::v;
+    constexpr int V = TVec<T>::V, VPR = kTT / V;  // vectors per tile row
+    __shared__ T tile[kTT][kTT + 1];
+    const int tid = threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.y * kTT, c0 = (int64_t)blockIdx.x * kTT;
+    const bool full = VEC && r0 + kTT <= M && c0 + kTT <= N;
+    // load: thread (row, vector) = (e / VPR, e % VPR), e = tid + 256 i
+#pragma unroll
+    for (int i = 0; i < kTT * VPR / 256; ++i) {
+        const int e = tid + 256 * i, r = e / VPR, cv = (e % VPR) * V;
+        vec x;
+        if (full) {
+            x = __builtin_nontemporal_load(reinterpret_cast<const vec *>(A + (r0 + r) * lda + c0 + cv));
+        } else {
+#pragma unroll
+            for (int q = 0; q < V; ++q)
+                x[q] = (r0 + r < M && c0 + cv + q < N) ? A[(r0 + r) * lda + c0 + cv + q] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < V; ++q) tile[cv + q][r] = x[q];
+    }
+    __syncthreads();
+    // store: row c0 + rr of B holds column c0 + rr of A
+#pragma unroll
+    for (int i = 0; i < kTT * VPR / 256; ++i) {
+        const int e = tid + 256 * i, rr = e / VPR, mv = (e % VPR) * V;
+        vec x;
+#pragma unroll
+        for (int q = 0; q < V; ++q) x[q] = tile[rr][mv + q];
+        if (full) {
+            __builtin_nontemporal_store(x, reinterpret_cast<vec *>(B + (c0 + rr) * ldb + r0 + mv));
+        } else {
+#pragma unroll
+            for (int q = 0; q < V; ++q)
+                if (c0 + rr < N && r0 + mv + q < M) B[(c0 + rr) * ldb + r0 + mv + q] = x[q];
+        }
+    }
+}
+
+__global__ void transpose_null_kernel() {}
+
+template <typename T>
+int transpose(int64_t M, int64_t N, const T *A, int64_t lda, T *B, int64_t ldb, void *stream)
+{
+    if (M < 0 || N < 0) return BLASDSP_ERR_INVALID;
+    if (M == 0 || N == 0) return BLASDSP_SUCCESS;
+    if (!A || !B || lda < N || ldb < M) return BLASDSP_ERR_INVALID;
+    // in place is not supported: the output must not overlap the input
+    const uintptr_t a0 = (uintptr_t)A, a1 = (uintptr_t)(A + (M - 1) * lda + N);
+    const uintptr_t b0 = (uintptr_t)B, b1 = (uintptr_t)(B + (N - 1) * ldb + M);
+    if (a0 < b1 && b0 < a1) return BLASDSP_ERR_INVALID;
+    const int64_t gx = (N + kTT - 1) / kTT, gy = (M + kTT - 1) / kTT;
+    if (gx > 0x7fffffffLL || gy > 65535) return BLASDSP_ERR_INVALID;
+    constexpr int V = TVec<T>::V;
+    const bool vec = a0 % 16 == 0 && b0 % 16 == 0 && lda % V == 0 && ldb % V == 0;
+    const dim3 grid((unsigned)gx, (unsigned)gy);
+    if (vec) hipLaunchKernelGGL((transpose_kernel<T, true>), grid, dim3(256), 0, (hipStream_t)stream, A, lda, B, ldb, M, N);
+    else hipLaunchKernelGGL((transpose_kernel<T, false>), grid, dim3(256), 0, (hipStream_t)stream, A, lda, B, ldb, M, N);
+    return hip_status(hipGetLastError());
+}
+
+int transpose_null(int64_t M, int64_t N, const void *A, const void *B, void *stream)
+{
+    if (M < 0 || N < 0 || (M > 0 && N > 0 && (!A || !B))) return BLASDSP_ERR_INVALID;
+    hipLaunchKernelGGL(transpose_null_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream);
+    return hip_status(hipGetLastError());
+}
+
+}  // namespace
+}  // namespace blasdsp
+
+extern "C" {
+
+int blasdsp_transpose_d(int64_t M, int64_t N, const double *A, int64_t lda, double *B, int64_t ldb, void *stream)
+{
+    return blasdsp::transpose<double>(M, N, A, lda, B, ldb, stream);
+}
+
+int blasdsp_transpose_s(int64_t M, int64_t N, const float *A, int64_t lda, float *B, int64_t ldb, void *stream)
+{
+    return blasdsp::transpose<float>(M, N, A, lda, B, ldb, stream);
+}
+
+int transpose_A2B(int64_t lda, int64_t ldb, int64_t M, int64_t N, const double *A, double *B, void *stream)
+{
+    return blasdsp::transpose<double>(M, N, A, lda, B, ldb, stream);
+}
+
+int transpose_A2B_float32(int64_t M, int64_t N, const float *A, float *B, void *stream)
+{
+    return blasdsp::transpose<float>(M, N, A, N, B, M, stream);
+}
+
+int transpose_16x16_dbuf(int64_t M, int64_t N, const double *A, double *B, void *stream)
+{
+    return blasdsp::transpose<double>(M, N, A, N, B, M, stream);
+}
+
+int transpose_float_32x32_dbuf(int64_t M, int64_t N, const float *A, float *B, void *stream)
+{
+    return blasdsp::transpose<float>(M, N, A, N, B, M, stream);
+}
+
+int transpose_null(int64_t M, int64_t N, const double *A, double *B, void *stream)
+{
+    return blasdsp::transpose_null(M, N, A, B, stream);
+}
+
+int transpose_float_null(int64_t M, int64_t N, const float *A, float *B, void *stream)
+{
+    return blasdsp::transpose_null(M, N, A, B, stream);
+}
+
+}  // extern "C"

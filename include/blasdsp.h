@@ -309,6 +309,28 @@ BLASDSP_API int blasdsp_tune_save(const char *path);
 BLASDSP_API int blasdsp_tune_load(const char *path);
 BLASDSP_API int blasdsp_tune_clear(void);
 
+/* ---- transpose: B = A^T, A row-major M x N, B N x M ----------------------
+ * HBM-bound (each element read once and written once).  The output must not
+ * overlap the input.  Statuses as for the GEMMs.
+ *   transpose/device_code/kernel.c:1167  transpose_16x16_dbuf(M, N, b_id, A, B)      FP64, packed
+ *   transpose/device_code/kernel.c:2002  transpose_float_32x32_dbuf(M, N, b_id, A, B) FP32, packed
+ *   transpose/device_code/kernel.c:1280,2110  transpose_null / transpose_float_null (empty launch)
+ *   dgemm_release_V1.1/include/dgemm_kernel.h:8  transpose_A2B(lda, ldb, M, N, b_id, A, B)
+ *   sgemm_release_V1.3.2/include/sgemm_kernel.h:8  transpose_A2B_float32(M, N, b_id, A, B)
+ * Packed means lda = N and ldb = M.  Barrier ids are dropped; a stream is added. */
+BLASDSP_API int transpose_16x16_dbuf(int64_t M, int64_t N, const double *A, double *B, void *stream);
+BLASDSP_API int transpose_float_32x32_dbuf(int64_t M, int64_t N, const float *A, float *B, void *stream);
+BLASDSP_API int transpose_null(int64_t M, int64_t N, const double *A, double *B, void *stream);
+BLASDSP_API int transpose_float_null(int64_t M, int64_t N, const float *A, float *B, void *stream);
+BLASDSP_API int transpose_A2B(int64_t lda, int64_t ldb, int64_t M, int64_t N, const double *A, double *B,
+                              void *stream);
+BLASDSP_API int transpose_A2B_float32(int64_t M, int64_t N, const float *A, float *B, void *stream);
+/* Any leading dimensions (lda >= N, ldb >= M); no counterpart in the reference. */
+BLASDSP_API int blasdsp_transpose_d(int64_t M, int64_t N, const double *A, int64_t lda, double *B,
+                                    int64_t ldb, void *stream);
+BLASDSP_API int blasdsp_transpose_s(int64_t M, int64_t N, const float *A, int64_t lda, float *B,
+                                    int64_t ldb, void *stream);
+
 /* ---- diagnostics ---------------------------------------------------------
  * Streams `bytes` (a multiple of 16, 16-byte aligned) from device memory p
  * with `blocks` workgroups and writes one word per workgroup to out (device,

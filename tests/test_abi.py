@@ -143,3 +143,17 @@ def test_one_hip_runtime_per_process():
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, check=True, timeout=300)
     libs = json.loads(out.stdout.strip().splitlines()[-1])
     assert len(libs) == 1, libs
+
+
+def test_transpose_rejects_bad_arguments(blas):
+    """Checked on the host before any HIP call."""
+    L = blas.lib()
+    buf = ctypes.create_string_buffer(1 << 12)
+    a = ctypes.addressof(buf)
+    assert L.blasdsp_transpose_s(-1, 4, a, 4, a + 2048, 4, None) == blas.ERR_INVALID
+    assert L.blasdsp_transpose_s(4, 4, a, 3, a + 2048, 4, None) == blas.ERR_INVALID      # lda < N
+    assert L.blasdsp_transpose_s(4, 4, a, 4, a + 2048, 3, None) == blas.ERR_INVALID      # ldb < M
+    assert L.blasdsp_transpose_d(4, 4, None, 4, a, 4, None) == blas.ERR_INVALID
+    assert L.blasdsp_transpose_s(8, 8, a, 8, a + 16, 8, None) == blas.ERR_INVALID        # overlapping
+    assert L.transpose_float_32x32_dbuf(0, 3, None, None, None) == blas.SUCCESS          # empty: no launch
+    assert L.transpose_A2B(2, 4, 4, 4, a, a + 2048, None) == blas.ERR_INVALID            # lda < N
