@@ -108,14 +108,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--workload", default="dgemm", choices=["dgemm", "skinny", "precision", "serving", "copy", "grouped",
-                                                        "transpose"])
+                                                        "transpose", "sgemm512", "sweep"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1"}))
         sys.exit(2)
     if args.workload != "dgemm":
         {"skinny": bench_skinny, "precision": bench_precision, "serving": bench_serving,
-         "copy": bench_copy, "grouped": bench_grouped, "transpose": bench_transpose}[args.workload](args)
+         "copy": bench_copy, "grouped": bench_grouped, "transpose": bench_transpose,
+         "sgemm512": bench_sgemm512, "sweep": bench_sweep}[args.workload](args)
         return
 
     import torch
@@ -447,6 +448,133 @@ def bench_transpose(args):
                   {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                    "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None},
                   ms_per_step=round(tot_ms / len(cases), 4))
+    print(json.dumps(out), flush=True)
+
+
+def bench_sgemm512(args):
+    """The reference's CPU-runnable case: SGEMM NN 512^3, lda = K, ldb = ldc = N,
+    alpha = beta = 1, U[-1,1) from seed 0.  GPU: the tile kernel through sgemm_NN
+    (launch-bound at this size).  CPU: the oracle's restatement of the
+    reference host GEMM (sgemm_demo.c:214-242, BLAS epilogue) on the whole
+    problem, min(16, cores) threads, median of 5; the two results are compared."""
+    import numpy as np
+    import torch
+    import __graft_entry__
+
+    blas = __graft_entry__.load_package()
+    orc = __graft_entry__.load_oracle()
+    n = 512
+    A = orc.matrix(n, n, orc.UNIFORM, 0, dtype=np.float32)
+    B = orc.matrix(n, n, orc.UNIFORM, 1, dtype=np.float32)
+    C0 = orc.matrix(n, n, orc.UNIFORM, 2, dtype=np.float32)
+    dA, dB = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+    dC = torch.from_numpy(C0).cuda()
+    stream = torch.cuda.current_stream()
+    L = blas.lib()
+    call = lambda: blas.check("sgemm_NN", L.sgemm_NN(n, n, n, n, n, n, dA.data_ptr(), dB.data_ptr(), dC.data_ptr(),
+                                                     1.0, 1.0, stream.cuda_stream))
+    # parity on one call from the original C
+    call()
+    torch.cuda.synchronize()
+    got = dC.cpu().numpy()
+    ref = orc.gemm_xp("N", "N", A, B, C0, 1.0, 1.0)
+    err = orc.normwise(got, ref)[0]
+    reps = max(20, args.steps * 10)
+    ms = _time_events(call, reps, stream)
+    flops = 2.0 * n ** 3
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        orc.set_threads(threads)
+        times = []
+        for _ in range(6):
+            C = np.array(C0)
+            t0 = time.perf_counter()
+            orc.gemm_blas_host(A, B, C, 1.0, 1.0)
+            times.append(time.perf_counter() - t0)
+        tmed = statistics.median(times[1:])
+        cpu_err = orc.normwise(C, ref)[0]
+        cpu = {"value": round(flops / tmed / 1e9, 3), "unit": "GFLOP/s", "cores": threads, "kind": "port",
+               "sample": f"the whole SGEMM NN 512^3 through the oracle's reference-tree sgemm (BLAS epilogue), "
+                         f"median of 5 runs of {tmed * 1e3:.2f} ms; normwise error vs the extended-precision "
+                         f"oracle {cpu_err:.2e}"}
+    out = _common(args, round(flops / ms / 1e6, 3), "GFLOP/s", "f32",
+                  {"workload": "SGEMM NN 512^3, lda=K, ldb=ldc=N, alpha=1, beta=1, U[-1,1) seed 0",
+                   "us_per_call": round(ms * 1e3, 2), "normwise_err_vs_oracle": err},
+                  {"bound": "mfma", "achieved": round(flops / ms / 1e9, 3), "peak": 157.3, "unit": "TFLOP/s",
+                   "frac": round(flops / ms / 1e9 / 157.3, 4), "traffic": None}, cpu, ms_per_step=round(ms, 4))
+    print(json.dumps(out), flush=True)
+
+
+def bench_sweep(args):
+    """The reference's published shape sweeps on this GPU: dgemm Fig. 1 (a)-(c)
+    (dgemm_release_V1.1/bin/run.sh: M, K, N varied around 3024 x 2048 x 1536,
+    all four modes) and irr_thinB_sgemm (irr_thinB_sgemm_release_V0.0/bin/run.sh:
+    M = 65536 with N = K in 16..96, and M = K = 20480 with N in 16..96)."""
+    import torch
+    import __graft_entry__
+
+    blas = __graft_entry__.load_package()
+    stream = torch.cuda.current_stream()
+    L = blas.lib()
+    reps = max(2, min(args.steps, 5))
+    dg = []
+    shapes = ([(m, 2048, 1536) for m in (1512, 3024, 6048, 12096, 24192, 48384)] +
+              [(3024, k, 1536) for k in (512, 1024, 4096, 8192, 16384)] +
+              [(3024, 20480, n) for n in (48, 96, 192)] + [(3024, 2048, n) for n in (384, 768, 3072, 6144)])
+    tot_f = tot_ms = 0.0
+    for M, K, N in shapes:
+        A = torch.empty((M, K), dtype=torch.float64, device="cuda")
+        AT = torch.empty((K, M), dtype=torch.float64, device="cuda")
+        B = torch.empty((K, N), dtype=torch.float64, device="cuda")
+        BT = torch.empty((N, K), dtype=torch.float64, device="cuda")
+        C = torch.empty((M, N), dtype=torch.float64, device="cuda")
+        for X, sd in ((A, 1), (AT, 2), (B, 3), (BT, 4), (C, 5)):
+            blas.fill(X, X.shape[0], X.shape[1], blas.UNIFORM, sd)
+        row = {"M": M, "K": K, "N": N}
+        for mode in MODES:
+            a = A if mode[0] == "N" else AT
+            b = B if mode[1] == "N" else BT
+            fn = getattr(L, "dgemm_" + mode)
+            lda, ldb = a.shape[1], b.shape[1]
+            ms = _time_events(lambda: blas.check(mode, fn(lda, ldb, N, M, N, K, a.data_ptr(), b.data_ptr(),
+                                                           C.data_ptr(), 1.0, 1.0, stream.cuda_stream)),
+                              reps, stream)
+            f = 2.0 * M * N * K
+            row[mode] = round(f / ms / 1e9, 2)
+            tot_f += f
+            tot_ms += ms
+        dg.append(row)
+        del A, AT, B, BT, C
+    tb = []
+    thin = [(65536, k, k) for k in (16, 32, 48, 64, 80, 96)] + [(20480, 20480, n) for n in (16, 32, 48, 64, 80, 96)]
+    for M, K, N in thin:
+        A = torch.empty((M, K), dtype=torch.float32, device="cuda")
+        AT = torch.empty((K, M), dtype=torch.float32, device="cuda")
+        B = torch.empty((K, N), dtype=torch.float32, device="cuda")
+        BT = torch.empty((N, K), dtype=torch.float32, device="cuda")
+        C = torch.empty((M, N), dtype=torch.float32, device="cuda")
+        for X, sd in ((A, 1), (AT, 2), (B, 3), (BT, 4), (C, 5)):
+            blas.fill(X, X.shape[0], X.shape[1], blas.UNIFORM, sd)
+        row = {"M": M, "K": K, "N": N}
+        nbytes = 4.0 * (M * K + K * N + 2 * M * N)
+        for mode in MODES:
+            a = A if mode[0] == "N" else AT
+            b = B if mode[1] == "N" else BT
+            fn = getattr(L, "irr_thinB_sgemm_" + mode)
+            lda, ldb = a.shape[1], b.shape[1]
+            ms = _time_events(lambda: blas.check(mode, fn(lda, ldb, N, M, N, K, a.data_ptr(), b.data_ptr(),
+                                                           C.data_ptr(), 1.0, 1.0, stream.cuda_stream)),
+                              reps, stream)
+            row[mode] = {"tflops": round(2.0 * M * N * K / ms / 1e9, 2), "gbs": round(nbytes / ms / 1e6, 1)}
+        tb.append(row)
+        del A, AT, B, BT, C
+    tf = tot_f / tot_ms / 1e9
+    out = _common(args, round(tf, 3), "TFLOP/s", "f64 (dgemm) + f32 (thinB)",
+                  {"workload": "reference shape sweeps: dgemm Fig. 1 (a)-(c), irr_thinB_sgemm N = K in 16..96",
+                   "dgemm_tflops_by_mode": dg, "irr_thinB_sgemm": tb},
+                  {"bound": "mfma", "achieved": round(tf, 3), "peak": 78.6, "unit": "TFLOP/s",
+                   "frac": round(tf / 78.6, 4), "traffic": None})
     print(json.dumps(out), flush=True)
 
 

@@ -130,7 +130,19 @@ int run(const Gemm<T> &g, int algo, int splitk, hipStream_t s)
     } else {
         return BLASDSP_ERR_INVALID;
     }
-    if (!use_skinny) return hip_status(launch_tile(g, tile, s));
+    if (!use_skinny) {
+        if (splitk <= 0) splitk = tile_pick_splitk(g, tile, num_cus());
+        if (splitk > 256) return BLASDSP_ERR_INVALID;
+        T *work = nullptr;
+        if (splitk > 1) {
+            const size_t bytes = (size_t)splitk * (size_t)g.M * (size_t)g.N * sizeof(T);
+            void *p = nullptr;
+            hipError_t e = workspace(s, bytes, &p);
+            if (e != hipSuccess) return hip_status(e);
+            work = (T *)p;
+        }
+        return hip_status(launch_tile(g, tile, splitk, work, s));
+    }
 
     if (splitk <= 0) splitk = skinny_pick_splitk(g, num_cus());
     if (splitk > 256) return BLASDSP_ERR_INVALID;

@@ -28,6 +28,9 @@
 //     share A and B panels in each XCD's L2;
 //   * epilogue C = alpha * acc + beta * C fused into the store (C not read
 //     when beta == 0).
+#include <algorithm>
+#include <cmath>
+
 #include "tile_body.h"
 
 namespace blasdsp {
@@ -63,17 +66,38 @@ struct PerCall {
 template <typename T, int BM, bool TA, bool TB>
 __global__ void __launch_bounds__((PerCall<T, BM>::Cf::THREADS))
 __attribute__((amdgpu_waves_per_eu(PerCall<T, BM>::Cf::WPE, PerCall<T, BM>::Cf::WPE)))
-tile_gemm_kernel(Gemm<T> g, int tiles_m, int tiles_n, bool a_vec, bool b_vec)
+tile_gemm_kernel(Gemm<T> g, int tiles_m, int tiles_n, bool a_vec, bool b_vec, int64_t kslice, T *W)
 {
     using Cf = typename PerCall<T, BM>::Cf;
     __shared__ __attribute__((aligned(16))) T lds[Cf::LDS_ELEMS];
     int tm, tn;
     tile_of_block(tiles_m, tiles_n, tm, tn);
     const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BM;
-    if (tile::full_tile<Cf>(g, m0, n0, a_vec, b_vec))
-        tile::tile_body<T, BM, TA, TB, Cf::NW, true>(g, m0, n0, a_vec, b_vec, lds);
+    // split-K: slice blockIdx.y of K writes its partial to W (one M x N per slice)
+    const int64_t kb = (int64_t)blockIdx.y * kslice, ke = min(g.K, kb + kslice);
+    T *w = W ? W + (int64_t)blockIdx.y * g.M * g.N : nullptr;
+    if (tile::full_tile<Cf>(g, m0, n0, kb, ke, a_vec, b_vec))
+        tile::tile_body<T, BM, TA, TB, Cf::NW, true>(g, m0, n0, a_vec, b_vec, lds, kb, ke, w);
     else
-        tile::tile_body<T, BM, TA, TB, Cf::NW, false>(g, m0, n0, a_vec, b_vec, lds);
+        tile::tile_body<T, BM, TA, TB, Cf::NW, false>(g, m0, n0, a_vec, b_vec, lds, kb, ke, w);
+}
+
+// C = alpha * sum_s W[s] + beta * C, slices added in order s = 0, 1, ...
+// (deterministic); C is not read when beta == 0.
+template <typename T>
+__global__ void __launch_bounds__(256) tile_reduce_kernel(const T *__restrict__ W, int splitk, T *__restrict__ C,
+                                                          int64_t M, int64_t N, int64_t ldc, T alpha, T beta)
+{
+    const int64_t total = M * N;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        T sum = W[e];
+        for (int q = 1; q < splitk; ++q) sum += W[(int64_t)q * total + e];
+        T *p = C + (e / N) * ldc + e % N;
+        T v = alpha * sum;
+        if (beta != T(0)) v += beta * *p;
+        *p = v;
+    }
 }
 
 template <typename T>
@@ -92,35 +116,89 @@ __global__ void null_kernel() {}
 
 }  // namespace
 
+// K per split-K slice: a whole number of K tiles.
+template <typename T>
+int64_t tile_kslice(int64_t K, int splitk)
+{
+    constexpr int64_t BK = tile::Cfg<T, 128>::BK;
+    const int64_t per = (K + splitk - 1) / splitk;
+    return (per + BK - 1) / BK * BK;
+}
+
 template <typename T, int BM>
-hipError_t launch_tile_bm(const Gemm<T> &g, hipStream_t s)
+hipError_t launch_tile_bm(const Gemm<T> &g, int splitk, T *work, hipStream_t s)
 {
     const int64_t tiles_m = (g.M + BM - 1) / BM;
     const int64_t tiles_n = (g.N + BM - 1) / BM;
     const int64_t nwg = tiles_m * tiles_n;
     if (nwg <= 0) return hipSuccess;
     if (nwg > 0x7fffffffLL) return hipErrorInvalidValue;
+    const int64_t kslice = splitk > 1 ? tile_kslice<T>(g.K, splitk) : g.K;
+    const int64_t slices = splitk > 1 ? (g.K + kslice - 1) / kslice : 1;
+    if (slices > 1 && !work) return hipErrorInvalidValue;
+    T *W = slices > 1 ? work : nullptr;
     const bool a_vec = tile::vec_ok(g.A, g.lda);
     const bool b_vec = tile::vec_ok(g.B, g.ldb);
-    const dim3 grid((unsigned)nwg), block(PerCall<T, BM>::Cf::THREADS);
+    const dim3 grid((unsigned)nwg, (unsigned)slices), block(PerCall<T, BM>::Cf::THREADS);
     const int tmi = (int)tiles_m, tni = (int)tiles_n;
     if (!g.ta && !g.tb)
-        hipLaunchKernelGGL((tile_gemm_kernel<T, BM, false, false>), grid, block, 0, s, g, tmi, tni, a_vec, b_vec);
+        hipLaunchKernelGGL((tile_gemm_kernel<T, BM, false, false>), grid, block, 0, s, g, tmi, tni, a_vec, b_vec, kslice, W);
     else if (!g.ta && g.tb)
-        hipLaunchKernelGGL((tile_gemm_kernel<T, BM, false, true>), grid, block, 0, s, g, tmi, tni, a_vec, b_vec);
+        hipLaunchKernelGGL((tile_gemm_kernel<T, BM, false, true>), grid, block, 0, s, g, tmi, tni, a_vec, b_vec, kslice, W);
     else if (g.ta && !g.tb)
-        hipLaunchKernelGGL((tile_gemm_kernel<T, BM, true, false>), grid, block, 0, s, g, tmi, tni, a_vec, b_vec);
+        hipLaunchKernelGGL((tile_gemm_kernel<T, BM, true, false>), grid, block, 0, s, g, tmi, tni, a_vec, b_vec, kslice, W);
     else
-        hipLaunchKernelGGL((tile_gemm_kernel<T, BM, true, true>), grid, block, 0, s, g, tmi, tni, a_vec, b_vec);
+        hipLaunchKernelGGL((tile_gemm_kernel<T, BM, true, true>), grid, block, 0, s, g, tmi, tni, a_vec, b_vec, kslice, W);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || slices == 1) return e;
+    const int64_t total = g.M * g.N;
+    int64_t blocks = (total + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    hipLaunchKernelGGL((tile_reduce_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, s, W, (int)slices, g.C, g.M,
+                       g.N, g.ldc, g.alpha, g.beta);
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_tile(const Gemm<T> &g, int bm, hipStream_t s)
+hipError_t launch_tile(const Gemm<T> &g, int bm, int splitk, T *work, hipStream_t s)
 {
-    if (bm == 64) return launch_tile_bm<T, 64>(g, s);
-    if (bm == 128) return launch_tile_bm<T, 128>(g, s);
+    if (bm == 64) return launch_tile_bm<T, 64>(g, splitk, work, s);
+    if (bm == 128) return launch_tile_bm<T, 128>(g, splitk, work, s);
     return hipErrorInvalidValue;
+}
+
+// Split-K for the tile kernel: the slice count 1 .. 16 (each slice at least
+// 16 K tiles deep) that minimises the estimated time
+//   ceil(units / slots) rounds x (one unit's time at full occupancy)
+//   + the reduction's HBM traffic ((2 S + 2) M N elements at 5 TB/s),
+// where units = C tiles x slices and slots = resident workgroups per CU x CUs.
+// A round is charged in full even when partly filled, so a GEMM whose tiles
+// already fill the slots keeps one slice.  Full-occupancy rates: FP64
+// 65 TFLOP/s, FP32 130 TFLOP/s (the tile kernel at 4096^3 / 8192^3).
+template <typename T>
+int tile_pick_splitk(const Gemm<T> &g, int bm, int num_cus)
+{
+    const int64_t tiles = ((g.M + bm - 1) / bm) * ((g.N + bm - 1) / bm);
+    const int64_t slots = (int64_t)(bm == 128 ? 2 : 4) * num_cus;
+    constexpr int64_t BK = tile::Cfg<T, 128>::BK;
+    const int64_t max_s = std::min<int64_t>(16, g.K / (16 * BK));
+    if (tiles <= 0 || max_s < 2) return 1;
+    const double rate = sizeof(T) == 8 ? 65e12 : 130e12, bw = 5e12;
+    const double unit_full = 2.0 * bm * bm * (double)g.K * (double)slots / rate;  // one slice = K
+    const double mn_bytes = (double)g.M * (double)g.N * sizeof(T);
+    int best = 1;
+    double best_t = std::ceil((double)tiles / (double)slots) * unit_full;
+    for (int64_t sk = 2; sk <= max_s; ++sk) {
+        const int64_t ks = tile_kslice<T>(g.K, (int)sk);
+        const int64_t slices = (g.K + ks - 1) / ks;
+        const double rounds = std::ceil((double)(tiles * slices) / (double)slots);
+        const double t = rounds * unit_full * (double)ks / (double)g.K + (2.0 * slices + 2.0) * mn_bytes / bw;
+        if (t < best_t * 0.97) {
+            best_t = t;
+            best = (int)slices;
+        }
+    }
+    return best;
 }
 
 template <typename T>
@@ -141,8 +219,10 @@ hipError_t launch_null(hipStream_t s)
     return hipGetLastError();
 }
 
-template hipError_t launch_tile<double>(const Gemm<double> &, int, hipStream_t);
-template hipError_t launch_tile<float>(const Gemm<float> &, int, hipStream_t);
+template hipError_t launch_tile<double>(const Gemm<double> &, int, int, double *, hipStream_t);
+template hipError_t launch_tile<float>(const Gemm<float> &, int, int, float *, hipStream_t);
+template int tile_pick_splitk<double>(const Gemm<double> &, int, int);
+template int tile_pick_splitk<float>(const Gemm<float> &, int, int);
 template hipError_t launch_scale<double>(const Gemm<double> &, hipStream_t);
 template hipError_t launch_scale<float>(const Gemm<float> &, hipStream_t);
 

@@ -22,8 +22,15 @@ struct Gemm {
 
 // MFMA tile kernel (gemm_tile.hip): any shape, any mode; bm = 64 or 128 is the
 // workgroup tile edge.
+// MFMA tile kernel with bm x bm tiles (64 or 128).  splitk > 1 cuts K into
+// that many slices (whole K tiles); their partials go to `work` (splitk * M *
+// N elements of T) and a second kernel adds them in order and applies the
+// epilogue.
 template <typename T>
-hipError_t launch_tile(const Gemm<T> &g, int bm, hipStream_t s);
+hipError_t launch_tile(const Gemm<T> &g, int bm, int splitk, T *work, hipStream_t s);
+// K slices the tile kernel would use for this shape on this GPU (1: none).
+template <typename T>
+int tile_pick_splitk(const Gemm<T> &g, int bm, int num_cus);
 
 // C = beta * C over M x N (beta == 0 writes zeros): K == 0 or alpha == 0.
 template <typename T>

@@ -138,23 +138,28 @@ struct Stager {
     }
 };
 
-// True when tile (m0, n0) can take the FULL body: both operands vector-
-// aligned, the tile inside C and K a whole number of K tiles.
+// True when tile (m0, n0) over K range [kb, ke) can take the FULL body: both
+// operands vector-aligned, the tile inside C and the range a whole number of
+// K tiles.
 template <typename Cf, typename T>
-__device__ __forceinline__ bool full_tile(const Gemm<T> &g, int64_t m0, int64_t n0, bool a_vec, bool b_vec)
+__device__ __forceinline__ bool full_tile(const Gemm<T> &g, int64_t m0, int64_t n0, int64_t kb, int64_t ke,
+                                          bool a_vec, bool b_vec)
 {
-    return a_vec && b_vec && m0 + Cf::BM <= g.M && n0 + Cf::BM <= g.N && g.K % Cf::BK == 0;
+    return a_vec && b_vec && m0 + Cf::BM <= g.M && n0 + Cf::BM <= g.N && (ke - kb) % Cf::BK == 0;
 }
 
-// One BM x BM tile of C at (m0, n0) with NW waves.  lds: Cfg::LDS_ELEMS
-// elements.  Every wave of the workgroup must call it (it contains
-// barriers).  FULL (see full_tile): no bounds checks anywhere.  The MFMA
-// sequence of every C element is the same for any NW and FULL, so all
+// One BM x BM tile of C at (m0, n0) with NW waves, over K range [kb, ke)
+// (kb a multiple of BK).  lds: Cfg::LDS_ELEMS elements.  Every wave of the
+// workgroup must call it (it contains barriers).  FULL (see full_tile): no
+// bounds checks anywhere.  W == nullptr: C = alpha acc + beta C; otherwise
+// the raw sums go to W (M x N, leading dimension N), a split-K partial.  The
+// MFMA sequence of every C element is the same for any NW and FULL, so those
 // variants give bit-identical results.
 template <typename T, int BM, bool TA, bool TB, int NW = 4, bool FULL = false>
 __device__ __forceinline__ void tile_body(const Gemm<T> &g, int64_t m0, int64_t n0, bool a_vec,
-                                          bool b_vec, T *lds)
+                                          bool b_vec, T *lds, int64_t kb = 0, int64_t ke = -1, T *W = nullptr)
 {
+    if (ke < 0) ke = g.K;
     using Cf = Cfg<T, BM, NW>;
     using acc_t = typename Mma<T>::acc_t;
     constexpr int BK = Cf::BK, SK = Cf::SK, SMN = Cf::SMN, MI = Cf::MI, NJ = Cf::NJ, BUF = Cf::BUF;
@@ -185,8 +190,8 @@ __device__ __forceinline__ void tile_body(const Gemm<T> &g, int64_t m0, int64_t 
 #pragma unroll
         for (int j = 0; j < NJ; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-    const int nk = (int)(FULL ? g.K / BK : (g.K + BK - 1) / BK);
-    load_tiles(0);
+    const int nk = (int)(FULL ? (ke - kb) / BK : (ke - kb + BK - 1) / BK);
+    load_tiles(kb);
     sa.store(stage(0, 0), tid);
     sb.store(stage(0, 1), tid);
     __syncthreads();
@@ -194,7 +199,7 @@ __device__ __forceinline__ void tile_body(const Gemm<T> &g, int64_t m0, int64_t 
     for (int kt = 0; kt < nk; ++kt) {
         const int cur = kt & 1;
         const bool more = kt + 1 < nk;
-        if (more) load_tiles((int64_t)(kt + 1) * BK);
+        if (more) load_tiles(kb + (int64_t)(kt + 1) * BK);
 
         const T *As = stage(cur, 0);
         const T *Bs = stage(cur, 1);
@@ -235,10 +240,14 @@ __device__ __forceinline__ void tile_body(const Gemm<T> &g, int64_t m0, int64_t 
             for (int r = 0; r < 4; ++r) {
                 const int64_t row = m0 + wm + i * 16 + Mma<T>::out_row(lk, r);
                 if (FULL || (row < g.M && col < g.N)) {
-                    T *p = g.C + row * g.ldc + col;
-                    T v = alpha * acc[i][j][r];
-                    if (beta != T(0)) v += beta * *p;
-                    *p = v;
+                    if (W) {
+                        W[row * g.N + col] = acc[i][j][r];
+                    } else {
+                        T *p = g.C + row * g.ldc + col;
+                        T v = alpha * acc[i][j][r];
+                        if (beta != T(0)) v += beta * *p;
+                        *p = v;
+                    }
                 }
             }
         }

@@ -8,9 +8,9 @@
 // limits are documented rather than tuned (dgemm_release_V1.1/readme.txt:11-25).
 //
 // Shape class: precision, transpose modes and round(log2) of M, N and K.
-// Candidates: the tile kernel with 64 and 128 tiles and, when the smaller of M
-// and N is at most 64, the skinny kernel with split-K factors 1 .. 64 (slices
-// of at least 256).  Each candidate runs once to warm up and then three times
+// Candidates: the tile kernel with 64 and 128 tiles, each whole or split into
+// 2, 4 or 8 K slices, and, when the smaller of M and N is at most 64, the
+// skinny kernel with split-K factors 1 .. 64 (slices of at least 256).  Each candidate runs once to warm up and then three times
 // between events on the caller's stream.  The table is a text file:
 //   blasdsp-tune-v1 <dtype> <ta> <tb> <log2 M> <log2 N> <log2 K> <algo> <tile> <splitk> <microseconds>
 #include <math.h>
@@ -48,7 +48,7 @@ template <typename T>
 hipError_t run_candidate(const Gemm<T> &g, const TuneChoice &c, T *work, hipStream_t s)
 {
     if (c.algo == BLASDSP_ALGO_SKINNY) return launch_skinny(g, c.splitk, work, s);
-    return launch_tile(g, c.tile, s);
+    return launch_tile(g, c.tile, c.splitk, work, s);
 }
 
 template <typename T>
@@ -56,15 +56,22 @@ int autotune(bool ta, bool tb, int64_t M, int64_t N, int64_t K, hipStream_t s, T
 {
     if (M <= 0 || N <= 0 || K <= 0) return BLASDSP_ERR_INVALID;
     std::vector<TuneChoice> cands;
-    cands.push_back(TuneChoice{BLASDSP_ALGO_TILE, 64, 1, 0});
-    cands.push_back(TuneChoice{BLASDSP_ALGO_TILE, 128, 1, 0});
-    const int64_t small = M < N ? M : N;
     int max_split = 1;
+    // tile kernel: 64 and 128 tiles, each whole or split into 2 .. 8 K slices
+    // of at least 256
+    for (int bm : {64, 128}) {
+        for (int sk = 1; sk <= 8; sk *= 2) {
+            if (sk > 1 && K / sk < 256) break;
+            cands.push_back(TuneChoice{BLASDSP_ALGO_TILE, bm, sk, 0});
+            max_split = sk > max_split ? sk : max_split;
+        }
+    }
+    const int64_t small = M < N ? M : N;
     if (small <= 64) {
         for (int sk = 1; sk <= 64; sk *= 2) {
             if (sk > 1 && K / sk < 256) break;
             cands.push_back(TuneChoice{BLASDSP_ALGO_SKINNY, 0, sk, 0});
-            max_split = sk;
+            max_split = sk > max_split ? sk : max_split;
         }
     }
     // Dense operands in their stored orientation, U[-1,1).
@@ -191,7 +198,7 @@ int blasdsp_tune_load(const char *path)
         if (n == EOF) break;
         const bool ok = n == 11 && strcmp(tag, "blasdsp-tune-v1") == 0 && (d == 0 || d == 1) &&
                         (ta == 0 || ta == 1) && (tb == 0 || tb == 1) && mb >= 0 && nb >= 0 && kb >= 0 &&
-                        ((algo == BLASDSP_ALGO_TILE && (tile == 64 || tile == 128)) ||
+                        ((algo == BLASDSP_ALGO_TILE && (tile == 64 || tile == 128) && sk >= 1 && sk <= 256) ||
                          (algo == BLASDSP_ALGO_SKINNY && sk >= 1 && sk <= 256));
         if (!ok) {
             st = BLASDSP_ERR_INVALID;

@@ -158,7 +158,10 @@ enum blasdsp_algo {
     BLASDSP_ALGO_TILE = 1,   /* MFMA tile kernel                               */
     BLASDSP_ALGO_SKINNY = 2  /* bandwidth-bound skinny kernel (+ split-K)      */
 };
-/* splitk <= 0: chosen by the library.  Only the skinny kernel splits K. */
+/* splitk: K slices, their partial products added in a fixed order by a
+ * second kernel (deterministic).  <= 0: chosen by the library — the tile
+ * kernel splits K only when its C tiles fill the GPU's workgroup slots
+ * poorly, the skinny kernel when its long side does not fill the GPU. */
 BLASDSP_API int blasdsp_dgemm_ex(char transA, char transB, int64_t M, int64_t N, int64_t K,
                      double alpha, const double *A, int64_t lda,
                      const double *B, int64_t ldb, double beta,
@@ -297,7 +300,7 @@ BLASDSP_API int blasdsp_serve(const blasdsp_request *reqs, int n, const blasdsp_
 typedef struct blasdsp_tune_choice {
     int algo;               /* BLASDSP_ALGO_TILE or BLASDSP_ALGO_SKINNY */
     int tile;               /* tile edge (64 or 128) for the tile kernel */
-    int splitk;             /* K slices for the skinny kernel */
+    int splitk;             /* K slices (1: no split) */
     double time_us;         /* measured time of one call */
 } blasdsp_tune_choice;
 BLASDSP_API int blasdsp_autotune(int dtype, char transA, char transB, int64_t M, int64_t N, int64_t K,

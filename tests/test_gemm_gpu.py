@@ -73,6 +73,43 @@ def test_skinny_kernel_modes(orc, blas, dev, prec, mode, shape):
     assert err <= (TOL64 if prec == "d" else tol32(K)), err
 
 
+@pytest.mark.parametrize("prec", ["d", "s"])
+@pytest.mark.parametrize("mode", ["NN", "NT", "TN", "TT"])
+@pytest.mark.parametrize("splitk", [2, 3, 5])
+@pytest.mark.parametrize("shape", [(200, 136, 1000), (257, 129, 1111), (64, 64, 2048), (130, 300, 515)])
+def test_tile_splitk(orc, blas, dev, prec, mode, splitk, shape):
+    """Tile kernel with K cut into slices (the last one ragged when K is not a
+    whole number of K tiles), partials added by the reduction kernel."""
+    M, N, K = shape
+    got, ref = run_case(orc, blas, dev, prec, mode[0], mode[1], M, N, K, 1.5, -0.5, blas.ALGO_TILE,
+                        splitk=splitk)
+    err, _ = orc.normwise(got, ref)
+    assert err <= (TOL64 if prec == "d" else tol32(K)), err
+
+
+def test_tile_splitk_deterministic_and_auto(orc, blas, dev):
+    """Split-K results are bit-reproducible, and the automatic choice (which
+    splits K for this shape: 25 tiles of 64 on a 256-CU GPU) matches the
+    oracle."""
+    import torch
+
+    M, N, K = 320, 320, 4096
+    A = orc.matrix(M, K, 0, 81)
+    B = orc.matrix(K, N, 0, 82)
+    C0 = orc.matrix(M, N, 0, 83)
+    ref = orc.gemm_xp("N", "N", A, B, C0, 1.0, 1.0)
+    outs = []
+    for splitk in (0, 0, 4, 4):
+        dC = to_dev(C0, dev)
+        blas.gemm("N", "N", to_dev(A, dev), to_dev(B, dev), dC, 1.0, 1.0, splitk=splitk)
+        torch.cuda.synchronize()
+        outs.append(dC.cpu().numpy())
+        err, _ = orc.normwise(outs[-1], ref)
+        assert err <= TOL64, (splitk, err)
+    np.testing.assert_array_equal(outs[0], outs[1])
+    np.testing.assert_array_equal(outs[2], outs[3])
+
+
 @pytest.mark.parametrize("splitk", [1, 2, 3, 7])
 def test_skinny_splitk(orc, blas, dev, splitk):
     for prec in "ds":

@@ -47,7 +47,8 @@ def test_grouped_matches_alone_and_oracle(orc, blas, dev, seed):
              for p, c in zip(probs, outs)]
     blas.gemm_grouped(descs)
     for p, c in zip(probs, alone):
-        blas.gemm(p["mode"][0], p["mode"][1], p["A"], p["B"], c, p["alpha"], p["beta"], algo=blas.ALGO_TILE)
+        blas.gemm(p["mode"][0], p["mode"][1], p["A"], p["B"], c, p["alpha"], p["beta"], algo=blas.ALGO_TILE,
+                  splitk=1)
     torch.cuda.synchronize()
     for i, (p, c, a) in enumerate(zip(probs, outs, alone)):
         got = np.ascontiguousarray(c.cpu().numpy())
