@@ -14,6 +14,12 @@ int hip_status(hipError_t e);
 int num_cus();
 // Grow-only scratch buffer owned by the library, one per (device, stream).
 hipError_t workspace(hipStream_t s, size_t bytes, void **out);
+// Frees the current device's scratch buffer of stream s (if any); the next
+// workspace() call for s allocates afresh.  The caller guarantees no work
+// using it is pending (hipFree also waits for the device).
+hipError_t workspace_release(hipStream_t s);
+// Destroys the cached group plan of stream s on the current device (if any).
+hipError_t grouped_plan_release(hipStream_t s);
 bool parse_trans(char t, bool &out);
 
 // Argument checks shared by every entry point (reference shape limits,

@@ -72,6 +72,19 @@ hipError_t workspace(hipStream_t s, size_t bytes, void **out)
     return hipSuccess;
 }
 
+hipError_t workspace_release(hipStream_t s)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    auto it = g_ws.find(std::make_pair(dev, s));
+    if (it == g_ws.end()) return hipSuccess;
+    if (it->second.ptr) e = hipFree(it->second.ptr);
+    g_ws.erase(it);
+    return e;
+}
+
 int num_cus()
 {
     static int cached[64] = {0};
@@ -327,6 +340,13 @@ int blasdsp_sgemm_fp8(char transA, char transB, int64_t M, int64_t N, int64_t K,
 {
     return blasdsp::split_entry(transA, transB, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, blasdsp::kSchemeFP8,
                                 stream);
+}
+
+int blasdsp_stream_release(void *stream)
+{
+    const hipError_t e1 = blasdsp::workspace_release((hipStream_t)stream);
+    const hipError_t e2 = blasdsp::grouped_plan_release((hipStream_t)stream);
+    return blasdsp::hip_status(e1 != hipSuccess ? e1 : e2);
 }
 
 int blasdsp_fill_f64(double *X, int64_t rows, int64_t cols, int64_t ld, int trans, int dist,

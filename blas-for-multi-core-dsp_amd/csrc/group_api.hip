@@ -209,6 +209,23 @@ std::mutex g_plan_mutex;
 std::map<std::pair<int, hipStream_t>, blasdsp_group_plan *> g_plans;
 
 }  // namespace
+
+hipError_t grouped_plan_release(hipStream_t s)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lock(g_plan_mutex);
+    auto it = g_plans.find(std::make_pair(dev, s));
+    if (it == g_plans.end()) return hipSuccess;
+    if (it->second) {
+        e = hipEventSynchronize(it->second->done);  // never recorded: returns at once
+        destroy_plan(it->second);
+    }
+    g_plans.erase(it);
+    return e;
+}
+
 }  // namespace blasdsp
 
 extern "C" {

@@ -221,7 +221,11 @@ void free_slot(Slot &s)
 {
     if (s.plan) blasdsp_group_plan_destroy(s.plan);
     if (s.done) (void)hipEventDestroy(s.done);
-    if (s.stream) (void)hipStreamDestroy(s.stream);
+    if (s.stream) {
+        // the per-request path may have given this stream a split-K workspace
+        (void)workspace_release(s.stream);
+        (void)hipStreamDestroy(s.stream);
+    }
     if (s.h_fill) (void)hipHostFree(s.h_fill);
     if (s.h_sum) (void)hipHostFree(s.h_sum);
     if (s.h_out) (void)hipHostFree(s.h_out);

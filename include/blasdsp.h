@@ -334,6 +334,14 @@ BLASDSP_API int blasdsp_transpose_d(int64_t M, int64_t N, const double *A, int64
 BLASDSP_API int blasdsp_transpose_s(int64_t M, int64_t N, const float *A, int64_t lda, float *B,
                                     int64_t ldb, void *stream);
 
+/* ---- per-stream resources (no counterpart in the reference) ---------------
+ * The library keeps, per (device, stream), a grow-only scratch buffer
+ * (split-K partials, split and FP8 operand planes) and the cached plan of
+ * blasdsp_gemm_grouped.  Call this before destroying a stream that was passed
+ * to the library, with no library work pending on it, to free them; using
+ * the stream again afterwards simply allocates anew. */
+BLASDSP_API int blasdsp_stream_release(void *stream);
+
 /* ---- diagnostics ---------------------------------------------------------
  * Streams `bytes` (a multiple of 16, 16-byte aligned) from device memory p
  * with `blocks` workgroups and writes one word per workgroup to out (device,

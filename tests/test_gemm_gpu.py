@@ -284,3 +284,33 @@ def test_skinny_full_size(orc, blas, dev, prec, kind):
     serr, _ = orc.normwise(np.ascontiguousarray(C.cpu().numpy()),
                            np.ascontiguousarray(C1.cpu().numpy()).astype(np.float64))
     assert serr <= tol, serr
+
+
+def test_stream_release_frees_scratch(orc, blas, dev):
+    """A split-K GEMM and a grouped call give a stream scratch memory and a
+    cached plan; blasdsp_stream_release frees them (twice is harmless), and
+    the stream works again afterwards."""
+    import torch
+
+    s = torch.cuda.Stream()
+    M, N, K = 512, 512, 8192
+    A = torch.rand((M, K), dtype=torch.float64, device=dev)
+    B = torch.rand((K, N), dtype=torch.float64, device=dev)
+    ref = (A.cpu() @ B.cpu()).numpy()
+    C = torch.zeros((M, N), dtype=torch.float64, device=dev)
+    Cg = torch.zeros((64, 64), dtype=torch.float64, device=dev)
+    with torch.cuda.stream(s):
+        blas.gemm("N", "N", A, B, C, 1.0, 0.0, algo=blas.ALGO_TILE, splitk=16, stream=s.cuda_stream)
+        blas.gemm_grouped([blas.desc("N", "N", A[:64, :64], B[:64, :64], Cg, 1.0, 0.0)], stream=s.cuda_stream)
+    s.synchronize()
+    assert np.allclose(C.cpu().numpy(), ref, rtol=1e-12)
+    assert np.allclose(Cg.cpu().numpy(), (A[:64, :64].cpu() @ B[:64, :64].cpu()).numpy(), rtol=1e-12)
+    blas.check("blasdsp_stream_release", blas.lib().blasdsp_stream_release(s.cuda_stream))
+    # releasing again (nothing left) is fine
+    blas.check("blasdsp_stream_release", blas.lib().blasdsp_stream_release(s.cuda_stream))
+    C.zero_()
+    with torch.cuda.stream(s):
+        blas.gemm("N", "N", A, B, C, 1.0, 0.0, algo=blas.ALGO_TILE, splitk=16, stream=s.cuda_stream)
+    s.synchronize()
+    assert np.allclose(C.cpu().numpy(), ref, rtol=1e-12)
+    blas.check("blasdsp_stream_release", blas.lib().blasdsp_stream_release(s.cuda_stream))
