@@ -317,6 +317,7 @@ hipError_t quantize(const float *X, int64_t ld, bool k_contig, int64_t R, int64_
                     uint8_t *Q, uint8_t *S, hipStream_t s)
 {
     const bool vec = ((uintptr_t)X % 16) == 0 && ld % 4 == 0;
+    if (KB > 65535 || Rp / 64 > 0x7fffffffLL) return hipErrorInvalidValue;  // grid limits
     if (k_contig) {
         const int64_t per_wave = 2 * kQuantBlocks, waves = (Rp * KB + per_wave - 1) / per_wave;
         const dim3 grid((unsigned)((waves + 3) / 4));
@@ -342,6 +343,7 @@ hipError_t launch_fp8_gemm(const Gemm<float> &g, void *work, hipStream_t s)
     hipError_t e = quantize(g.A, g.lda, !g.ta, g.M, g.K, L.Mp, L.KB, L.Qa, L.Sa, s);
     if (e == hipSuccess) e = quantize(g.B, g.ldb, g.tb, g.N, g.K, L.Np, L.KB, L.Qb, L.Sb, s);
     if (e != hipSuccess) return e;
+    if ((L.Mp / kTile) * (L.Np / kTile) > 0x7fffffffLL) return hipErrorInvalidValue;
     const int tiles_m = (int)(L.Mp / kTile), tiles_n = (int)(L.Np / kTile);
     hipLaunchKernelGGL(fp8_gemm_kernel, dim3((unsigned)(tiles_m * tiles_n)), dim3(256), 0, s, L.Qa, L.Qb,
                        (const uint32_t *)L.Sa, (const uint32_t *)L.Sb, L.Kp, L.Mp, L.Np, tiles_m, tiles_n, g.C,

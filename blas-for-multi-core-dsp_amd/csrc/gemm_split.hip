@@ -417,6 +417,7 @@ template <int NP, bool FP16>
 hipError_t launch_split(const float *X, int64_t ld, bool kc, int64_t R, int64_t K, uint16_t *planes,
                         int64_t Rp, int64_t Kp, const float *scale, hipStream_t s)
 {
+    if (Rp / 64 > 65535 || Kp / 64 > 0x7fffffffLL) return hipErrorInvalidValue;  // grid limits
     const dim3 grid((unsigned)(Kp / 64), (unsigned)(Rp / 64));
     const bool vec = ((uintptr_t)X % 16) == 0 && ld % 4 == 0;
     const int64_t pe = Rp * Kp;

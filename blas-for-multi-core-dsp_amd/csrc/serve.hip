@@ -305,8 +305,9 @@ extern "C" int blasdsp_serve(const blasdsp_request *reqs, int n, const blasdsp_s
 {
     using namespace blasdsp;
     if (n < 0 || (n > 0 && (!reqs || !results)) || !cfg) return BLASDSP_ERR_INVALID;
-    if (cfg->memory_budget <= 0 || cfg->max_batch <= 0 || cfg->max_inflight <= 0 || cfg->max_inflight > 16 ||
-        cfg->time_scale < 0)
+    // max_batch bounds the grouped fill launch (3 jobs per request in grid.y <= 65535)
+    if (cfg->memory_budget <= 0 || cfg->max_batch <= 0 || cfg->max_batch > 16384 || cfg->max_inflight <= 0 ||
+        cfg->max_inflight > 16 || cfg->time_scale < 0)
         return BLASDSP_ERR_INVALID;
     const bool grouped = cfg->grouped != 0;
     const int max_batch = grouped ? cfg->max_batch : 1;

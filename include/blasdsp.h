@@ -269,7 +269,7 @@ typedef struct blasdsp_request_result {
 
 typedef struct blasdsp_serve_config {
     int64_t memory_budget;  /* HBM bytes for resident operands */
-    int max_batch;          /* requests per grouped launch */
+    int max_batch;          /* requests per grouped launch, 1..16384 */
     int max_inflight;       /* launch slots (streams) in flight at once, 1..16 */
     int grouped;            /* 1: grouped launches; 0: one launch per request */
     double time_scale;      /* multiplies arrival times; 0: all arrive at once */

@@ -103,6 +103,8 @@ def test_serve_rejects_bad_config(blas):
     assert blas.lib().blasdsp_serve(req, 1, cfg, res, None, None) == blas.ERR_INVALID
     cfg = blas.ServeConfig(1 << 20, 8, 0, 1, 0.0)  # no launch slot
     assert blas.lib().blasdsp_serve(req, 1, cfg, res, None, None) == blas.ERR_INVALID
+    cfg = blas.ServeConfig(1 << 20, 16385, 2, 1, 0.0)  # batch beyond the fill launch's grid
+    assert blas.lib().blasdsp_serve(req, 1, cfg, res, None, None) == blas.ERR_INVALID
 
 
 def test_tune_table_load_query_save(blas, tmp_path):
