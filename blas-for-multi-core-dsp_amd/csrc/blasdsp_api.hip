@@ -115,7 +115,7 @@ bool skinny_shape(const Gemm<T> &g)
 {
     const int64_t small = g.M <= g.N ? g.M : g.N;
     const int64_t large = g.M <= g.N ? g.N : g.M;
-    return small <= 64 && large >= 2 * small;
+    return small <= 64 && large >= 2 * small && skinny_fits(g);
 }
 
 template <typename T>
@@ -123,17 +123,16 @@ int run(const Gemm<T> &g, int algo, int splitk, hipStream_t s)
 {
     if (g.M == 0 || g.N == 0) return BLASDSP_SUCCESS;
     if (g.K == 0 || g.alpha == T(0)) return hip_status(launch_scale(g, s));
-    const int64_t small = g.M <= g.N ? g.M : g.N;
     // 64 x 64 tiles when 128 x 128 tiles would leave most CUs idle
     const int64_t t128 = ((g.M + 127) / 128) * ((g.N + 127) / 128);
     int tile = t128 < num_cus() ? 64 : 128;
     bool use_skinny;
     if (algo == BLASDSP_ALGO_TILE) use_skinny = false;
-    else if (algo == BLASDSP_ALGO_SKINNY) use_skinny = small <= 64;  // larger: tile kernel
+    else if (algo == BLASDSP_ALGO_SKINNY) use_skinny = skinny_fits(g);  // else: tile kernel
     else if (algo == BLASDSP_ALGO_AUTO) {
         TuneChoice c;
         if (tune_lookup(sizeof(T) == 8, g.ta, g.tb, g.M, g.N, g.K, &c) &&
-            (c.algo == BLASDSP_ALGO_TILE || small <= 64)) {
+            (c.algo == BLASDSP_ALGO_TILE || skinny_fits(g))) {
             use_skinny = c.algo == BLASDSP_ALGO_SKINNY;
             tile = c.tile;
             if (splitk <= 0) splitk = c.splitk;

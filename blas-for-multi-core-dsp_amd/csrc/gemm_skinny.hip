@@ -163,27 +163,134 @@ __device__ __forceinline__ V stream_load(const V *p)
     return __builtin_nontemporal_load(p);
 }
 
-template <typename T, int MT, bool S_KC, bool L_KC, bool MASK>
-__device__ __forceinline__ void load_block(Block<T, MT, S_KC, L_KC> &b, const SkinnyArgs<T> &a,
-                                           int64_t n0, int64_t k0, int64_t kend, int c, int g)
+// Byte offsets of one lane from the wave-uniform bases of the unmasked loads.
+// They do not change from block to block: the loads are buffer loads whose
+// resource (the base) is rebuilt in scalar registers per block, so each
+// operand needs one offset register in total (the host checks that the
+// offsets fit: skinny_fits).
+template <int MT>
+struct LaneOff {
+    uint32_t l;        // streamed operand
+    uint32_t s[MT];    // small operand, per 16-row tile
+};
+
+// k_of(k0, g, j) = k0 + kgrp(g) + kstep(j): the lane-group part and the step
+// part of the permuted k index.
+template <typename T, bool L_KC>
+__device__ __forceinline__ int kgrp(int g)
+{
+    return L_KC ? g * Blk<T>::V : g * Blk<T>::RUN;
+}
+template <typename T, bool L_KC>
+__host__ __device__ constexpr int kstep(int j)
+{
+    return L_KC ? (j / Blk<T>::V) * 4 * Blk<T>::V + j % Blk<T>::V : j;
+}
+
+template <typename T, int MT, int SM, bool L_KC>
+__device__ __forceinline__ LaneOff<MT> lane_offsets(const SkinnyArgs<T> &a, int c, int g)
+{
+    constexpr int V = Blk<T>::V, RUN = Blk<T>::RUN;
+    constexpr int64_t s = sizeof(T);
+    const int lane = g * 16 + c;
+    LaneOff<MT> o;
+    if (L_KC) o.l = (uint32_t)(((int64_t)(lane >> 3) * a.ldl + (lane & 7) * V) * s);
+    else o.l = (uint32_t)(((int64_t)g * RUN * a.ldl + V * c) * s);
+    const int64_t kg = kgrp<T, L_KC>(g);
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+        // Rows past Ms read row Ms - 1.  An MFMA output row depends only on
+        // the same row of the A operand, and output rows past Ms are never
+        // stored, so those rows need no zeroing.
+        const int64_t ms = mt * 16 + c;
+        const int64_t mr = ms < a.Ms ? ms : a.Ms - 1;
+        o.s[mt] = (uint32_t)((SM != 0 ? mr * a.lds + kg : kg * a.lds + mr) * s);
+    }
+    return o;
+}
+
+// Buffer loads from a wave-uniform base: raw buffer, no stride, records up to
+// 2^31 bytes (lane offsets stay below that), the CDNA data-format word.
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const T *base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(base), (short)0, 0x7fffffff, 0x00020000);
+}
+constexpr int kAuxNT = 2;  // cache policy: non-temporal
+template <typename T, int AUX>
+__device__ __forceinline__ typename SK<T>::vec_t bload_vec(const T *base, uint32_t off)
+{
+    return __builtin_bit_cast(typename SK<T>::vec_t, __builtin_amdgcn_raw_buffer_load_b128(rsrc(base), off, 0, AUX));
+}
+template <typename T>
+__device__ __forceinline__ T bload(const T *base, uint32_t off)
+{
+    if constexpr (sizeof(T) == 8)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(rsrc(base), off, 0, 0));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rsrc(base), off, 0, 0));
+}
+
+// One full K block (k0 .. k0 + KB of a panel inside Nl) with no bounds
+// checks and no lane branches: every block issues the same loads, so the
+// wait before a block's MFMAs counts exactly the loads issued after it.
+// n0 and k0 are wave-uniform.
+template <typename T, int MT, int SM, bool L_KC>
+__device__ __forceinline__ void load_full(Block<T, MT, SM != 0, L_KC> &b, const SkinnyArgs<T> &a,
+                                          const LaneOff<MT> &o, int64_t n0, int64_t k0)
+{
+    using B_ = Blk<T>;
+    constexpr int V = B_::V;
+    if (L_KC) {
+        // vector i: 16 bytes of row n0 + 8 i + lane / 8, chunk lane % 8; each
+        // instruction reads whole 128-byte lines of 8 rows, which
+        // stage_block() turns into MFMA operands through LDS
+#pragma unroll
+        for (int i = 0; i < B_::NLV; ++i) b.l[i] = bload_vec<T, kAuxNT>(a.L + (n0 + 8 * i) * a.ldl + k0, o.l);
+    } else {
+        // step j: row k_of(k0, g, j), columns n0 + h*16V + V*c .. + V-1
+#pragma unroll
+        for (int j = 0; j < B_::NSTEP; ++j)
+#pragma unroll
+            for (int h = 0; h < 4 / V; ++h)
+                b.l[j * (4 / V) + h] = bload_vec<T, kAuxNT>(a.L + (k0 + j) * a.ldl + n0 + h * 16 * V, o.l);
+    }
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+        if (SM != 0) {
+#pragma unroll
+            for (int q = 0; q < B_::RV; ++q) {
+                const T *base = a.S + k0 + kstep<T, L_KC>(q * V);
+                if constexpr (SM == 2) {
+                    b.s_run[mt][q] = bload_vec<T, 0>(base, o.s[mt]);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < V; ++e) b.s_run[mt][q][e] = bload<T>(base + e, o.s[mt]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < B_::NSTEP; ++j)
+                b.s_scl[mt][j] = bload<T>(a.S + (k0 + kstep<T, L_KC>(j)) * a.lds, o.s[mt]);
+        }
+    }
+}
+
+// A ragged block (partial panel or partial K block), every element
+// bounds-checked, L read straight into the operand layout.  S access mode SM:
+// 0 = S stored m-contiguous (one scalar per step), 1 = K-contiguous read as
+// scalars (S not 16-byte aligned, or lds not a multiple of the vector), 2 =
+// K-contiguous read as 16-byte vectors.
+template <typename T, int MT, int SM, bool L_KC>
+__device__ __forceinline__ void load_tail(Block<T, MT, SM != 0, L_KC> &b, const SkinnyArgs<T> &a,
+                                          int64_t n0, int64_t k0, int64_t kend, int c, int g)
 {
     using vec_t = typename SK<T>::vec_t;
     using B_ = Blk<T>;
     constexpr int V = B_::V;
     // ---- L ----
-    if (L_KC && !MASK) {
-        // staged: vector i of the lane is 16 bytes of row n0 + 8 i + lane / 8,
-        // chunk lane % 8, so each instruction reads whole 128-byte lines of 8
-        // rows; stage_block() turns these into MFMA operands through LDS.
-        const int lane = g * 16 + c;
-#pragma unroll
-        for (int i = 0; i < B_::NLV; ++i) {
-            const int64_t row = n0 + 8 * i + (lane >> 3);
-            b.l[i] = stream_load(reinterpret_cast<const vec_t *>(a.L + row * a.ldl + k0 + (lane & 7) * V));
-        }
-    } else if (L_KC) {
+    if (L_KC) {
         // column tile t: row n0 + 16t + c, vector q: k_of(k0, g, qV) .. + V-1
-        // (the four lane groups read 64 contiguous bytes of the row)
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
 #pragma unroll
@@ -191,9 +298,7 @@ __device__ __forceinline__ void load_block(Block<T, MT, S_KC, L_KC> &b, const Sk
                 const int64_t row = n0 + 16 * t + c;
                 const int64_t kk = k_of<T, true>(k0, g, q * V);
                 vec_t v;
-                if (!MASK) {
-                    v = *reinterpret_cast<const vec_t *>(a.L + row * a.ldl + kk);
-                } else if (a.l_vec && row < a.Nl && kk + V <= kend) {
+                if (a.l_vec && row < a.Nl && kk + V <= kend) {
                     v = *reinterpret_cast<const vec_t *>(a.L + row * a.ldl + kk);
                 } else {
 #pragma unroll
@@ -212,9 +317,7 @@ __device__ __forceinline__ void load_block(Block<T, MT, S_KC, L_KC> &b, const Sk
                 const int64_t kk = k_of<T, false>(k0, g, j);
                 const int64_t col = n0 + h * 16 * V + V * c;
                 vec_t v;
-                if (!MASK) {
-                    v = stream_load(reinterpret_cast<const vec_t *>(a.L + kk * a.ldl + col));
-                } else if (a.l_vec && kk < kend && col + V <= a.Nl) {
+                if (a.l_vec && kk < kend && col + V <= a.Nl) {
                     v = stream_load(reinterpret_cast<const vec_t *>(a.L + kk * a.ldl + col));
                 } else {
 #pragma unroll
@@ -225,16 +328,16 @@ __device__ __forceinline__ void load_block(Block<T, MT, S_KC, L_KC> &b, const Sk
             }
         }
     }
-    // ---- S (always bounds-checked on Ms; cheap, L1/L2 resident) ----
+    // ---- S ----
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
         const int64_t ms = mt * 16 + c;
-        if (S_KC) {
+        if (SM != 0) {
 #pragma unroll
             for (int q = 0; q < B_::RV; ++q) {
                 const int64_t kk = k_of<T, L_KC>(k0, g, q * V);
                 vec_t v;
-                if (ms < a.Ms && a.s_vec && (!MASK || kk + V <= kend)) {
+                if (SM == 2 && ms < a.Ms && kk + V <= kend) {
                     v = *reinterpret_cast<const vec_t *>(a.S + ms * a.lds + kk);
                 } else {
 #pragma unroll
@@ -247,7 +350,7 @@ __device__ __forceinline__ void load_block(Block<T, MT, S_KC, L_KC> &b, const Sk
 #pragma unroll
             for (int j = 0; j < B_::NSTEP; ++j) {
                 const int64_t kk = k_of<T, L_KC>(k0, g, j);
-                b.s_scl[mt][j] = (ms < a.Ms && (!MASK || kk < kend)) ? a.S[kk * a.lds + ms] : T(0);
+                b.s_scl[mt][j] = (ms < a.Ms && kk < kend) ? a.S[kk * a.lds + ms] : T(0);
             }
         }
     }
@@ -297,20 +400,43 @@ __device__ __forceinline__ void mma_block(typename SK<T>::acc_t (&acc)[MT][4],
             for (int t = 0; t < 4; ++t) acc[mt][t] = SK<T>::mma(b.sop(mt, j), b.lop(t, j), acc[mt][t]);
 }
 
-// Waves per SIMD the register allocation must allow: the kernel streams from
-// HBM, so more resident waves means more bytes in flight per CU.
+// One step of the ring: multiply block b.  The scheduling barrier keeps the
+// loads of later steps below this step's MFMAs: a ring slot refilled by a
+// later step is a new value to the compiler, and hoisting its loads would
+// keep one more block live than the ring holds.
+template <typename T, int MT, bool S_KC, bool L_KC>
+__device__ __forceinline__ void consume_block(typename SK<T>::acc_t (&acc)[MT][4], Block<T, MT, S_KC, L_KC> &b,
+                                              T *w, int c, int g)
+{
+    if constexpr (L_KC) stage_block<T, MT, S_KC>(b, w, c, g);
+    mma_block<T, MT, S_KC, L_KC>(acc, b);
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+// Waves per SIMD the register allocation must allow, and DEPTH: K blocks
+// whose loads are in flight while a wave multiplies the current one.  Resident
+// waves times DEPTH sets the bytes in flight per CU.  Measured on MI355X at
+// K = 8192, long side 65536 (all modes): one wave per SIMD with a ring of 4
+// (16 rows) or 3 (32 rows) blocks streams as fast as or up to 4.5 % faster
+// than 2-4 waves with rings of 1-3, and ring depths 3-8 are alike.  One wave
+// per SIMD also needs no K split at that size.  64-row FP32 keeps 2 waves, 1
+// block.
 template <typename T, int MT>
 struct SkinnyOcc {
-    static constexpr int WPE = sizeof(T) == 8 ? (MT == 1 ? 3 : (MT == 2 ? 2 : 1)) : (MT == 1 ? 4 : (MT == 2 ? 3 : 2));
+    static constexpr int WPE = MT <= 2 ? 1 : (sizeof(T) == 8 ? 1 : 2);
+    static constexpr int DEPTH = MT == 1 ? 4 : (MT == 2 ? 3 : 1);
 };
 
-template <typename T, int MT, bool S_KC, bool L_KC, bool TRANS_OUT>
+template <typename T, int MT, int SM, bool L_KC, bool TRANS_OUT>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(SkinnyOcc<T, MT>::WPE, SkinnyOcc<T, MT>::WPE)))
 skinny_kernel(SkinnyArgs<T> a)
 {
+    constexpr bool S_KC = SM != 0;
     using acc_t = typename SK<T>::acc_t;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the wave index is made wave-uniform, so n0 and the load bases below
+    // live in scalar registers
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int c = lane & 15, g = lane >> 4;
     const int64_t n0 = ((int64_t)blockIdx.x * kWaves + wave) * kPanel;
     if (n0 >= a.Nl) return;  // no barriers in this kernel: idle waves may leave
@@ -332,25 +458,42 @@ skinny_kernel(SkinnyArgs<T> a)
     __shared__ __attribute__((aligned(16))) T stage[L_KC ? kWaves * Stage<T>::ELEMS : 1];
     T *const w = stage + (L_KC ? wave * Stage<T>::ELEMS : 0);
 
-    // Full blocks: unmasked L loads, register double buffer.
-    int64_t k0 = kb;
-    if (k0 < kfull) {
-        Block<T, MT, S_KC, L_KC> cur, nxt;
-        load_block<T, MT, S_KC, L_KC, false>(cur, a, n0, k0, ke, c, g);
-        for (; k0 + KB < kfull; k0 += KB) {
-            load_block<T, MT, S_KC, L_KC, false>(nxt, a, n0, k0 + KB, ke, c, g);
-            if constexpr (L_KC) stage_block<T, MT, S_KC>(cur, w, c, g);
-            mma_block<T, MT, S_KC, L_KC>(acc, cur);
-            cur = nxt;
+    // Full blocks: unmasked loads into a ring of D + 1 register blocks.
+    // Block j lives in b[j % (D + 1)]; while block j is multiplied, the loads
+    // of blocks j + 1 .. j + D are in flight.  The steady-state loop runs
+    // whole rounds of D + 1 blocks, so every ring index is a compile-time
+    // constant and no block is copied between registers.
+    constexpr int D = SkinnyOcc<T, MT>::DEPTH;
+    const int64_t nb = (kfull - kb) / KB;
+    if (nb > 0) {
+        const LaneOff<MT> o = lane_offsets<T, MT, SM, L_KC>(a, c, g);
+        Block<T, MT, S_KC, L_KC> b[D + 1];
+#pragma unroll
+        for (int u = 0; u < D; ++u)
+            if (u < nb) load_full<T, MT, SM, L_KC>(b[u], a, o, n0, kb + u * KB);
+        int64_t i = 0;
+        for (; i + 2 * D < nb; i += D + 1) {
+#pragma unroll
+            for (int u = 0; u <= D; ++u) {
+                load_full<T, MT, SM, L_KC>(b[(u + D) % (D + 1)], a, o, n0, kb + (i + u + D) * KB);
+                consume_block<T, MT, S_KC, L_KC>(acc, b[u], w, c, g);
+            }
         }
-        if constexpr (L_KC) stage_block<T, MT, S_KC>(cur, w, c, g);
-        mma_block<T, MT, S_KC, L_KC>(acc, cur);
-        k0 = kfull;
+        // the last nb - i <= 2D blocks; blocks i .. i + D - 1 are loaded
+#pragma unroll
+        for (int u = 0; u < 2 * D; ++u) {
+            if (i + u < nb) {
+                if (i + u + D < nb)
+                    load_full<T, MT, SM, L_KC>(b[(u + D) % (D + 1)], a, o, n0, kb + (i + u + D) * KB);
+                consume_block<T, MT, S_KC, L_KC>(acc, b[u % (D + 1)], w, c, g);
+            }
+        }
     }
+    int64_t k0 = kfull;
     // Ragged tail (partial panel or partial K block): masked loads.
     for (; k0 < ke; k0 += KB) {
         Block<T, MT, S_KC, L_KC> b;
-        load_block<T, MT, S_KC, L_KC, true>(b, a, n0, k0, ke, c, g);
+        load_tail<T, MT, SM, L_KC>(b, a, n0, k0, ke, c, g);
         mma_block<T, MT, S_KC, L_KC>(acc, b);
     }
 
@@ -399,11 +542,11 @@ __global__ void skinny_reduce_kernel(SkinnyArgs<T> a)
     }
 }
 
-template <typename T, int MT, bool S_KC, bool L_KC, bool TRANS_OUT>
+template <typename T, int MT, int SM, bool L_KC, bool TRANS_OUT>
 hipError_t launch4(const SkinnyArgs<T> &a, hipStream_t s)
 {
     const int64_t wgs = (a.Nl + kWaves * kPanel - 1) / (kWaves * kPanel);
-    hipLaunchKernelGGL((skinny_kernel<T, MT, S_KC, L_KC, TRANS_OUT>), dim3((unsigned)wgs, a.splitk),
+    hipLaunchKernelGGL((skinny_kernel<T, MT, SM, L_KC, TRANS_OUT>), dim3((unsigned)wgs, a.splitk),
                        dim3(kWaves * 64), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || a.splitk == 1) return e;
@@ -414,15 +557,19 @@ hipError_t launch4(const SkinnyArgs<T> &a, hipStream_t s)
     return hipGetLastError();
 }
 
+template <typename T, int MT, int SM>
+hipError_t launch_sm(const SkinnyArgs<T> &a, bool l_kc, bool trans_out, hipStream_t s)
+{
+    if (l_kc) return trans_out ? launch4<T, MT, SM, true, true>(a, s) : launch4<T, MT, SM, true, false>(a, s);
+    return trans_out ? launch4<T, MT, SM, false, true>(a, s) : launch4<T, MT, SM, false, false>(a, s);
+}
+
 template <typename T, int MT>
 hipError_t launch_mt(const SkinnyArgs<T> &a, bool s_kc, bool l_kc, bool trans_out, hipStream_t s)
 {
-    if (s_kc) {
-        if (l_kc) return trans_out ? launch4<T, MT, true, true, true>(a, s) : launch4<T, MT, true, true, false>(a, s);
-        return trans_out ? launch4<T, MT, true, false, true>(a, s) : launch4<T, MT, true, false, false>(a, s);
-    }
-    if (l_kc) return trans_out ? launch4<T, MT, false, true, true>(a, s) : launch4<T, MT, false, true, false>(a, s);
-    return trans_out ? launch4<T, MT, false, false, true>(a, s) : launch4<T, MT, false, false, false>(a, s);
+    if (!s_kc) return launch_sm<T, MT, 0>(a, l_kc, trans_out, s);
+    if (!a.s_vec) return launch_sm<T, MT, 1>(a, l_kc, trans_out, s);
+    return launch_sm<T, MT, 2>(a, l_kc, trans_out, s);
 }
 
 // Operand roles for a validated GEMM.  shortA when M <= N, thinB otherwise.
@@ -453,6 +600,22 @@ void roles(const Gemm<T> &g, SkinnyArgs<T> &a, bool &s_kc, bool &l_kc, bool &tra
 }  // namespace
 
 template <typename T>
+bool skinny_fits(const Gemm<T> &g)
+{
+    SkinnyArgs<T> a{};
+    bool s_kc, l_kc, trans_out;
+    roles(g, a, s_kc, l_kc, trans_out);
+    if (a.Ms > 64 || a.Ms <= 0 || a.Nl <= 0) return false;
+    // largest lane offsets of lane_offsets(), in bytes (rows of S up to 63)
+    constexpr double V = SK<T>::V, RUN = Blk<T>::RUN, s = sizeof(T);
+    const double kg = 3 * (l_kc ? V : RUN);
+    const double lmax = l_kc ? (7.0 * (double)a.ldl + 7 * V) * s : (3 * RUN * (double)a.ldl + 15 * V) * s;
+    const double smax = s_kc ? (63.0 * (double)a.lds + kg) * s : (kg * (double)a.lds + 63) * s;
+    // a 16-byte access at the offset stays inside the buffer resource's 2^31 - 1 bytes
+    return lmax + 16 < 2147483647.0 && smax + 16 < 2147483647.0;
+}
+
+template <typename T>
 int skinny_pick_splitk(const Gemm<T> &g, int num_cus)
 {
     // One full round of resident workgroups: a workgroup is one wave per
@@ -473,10 +636,10 @@ int skinny_pick_splitk(const Gemm<T> &g, int num_cus)
 template <typename T>
 hipError_t launch_skinny(const Gemm<T> &g, int splitk, T *work, hipStream_t s)
 {
+    if (!skinny_fits(g)) return hipErrorInvalidValue;
     SkinnyArgs<T> a{};
     bool s_kc, l_kc, trans_out;
     roles(g, a, s_kc, l_kc, trans_out);
-    if (a.Ms > 64 || a.Ms <= 0 || a.Nl <= 0) return hipErrorInvalidValue;
     if (splitk < 1) splitk = 1;
     constexpr int KB = Blk<T>::KB;
     int64_t kslice = (g.K + splitk - 1) / splitk;
@@ -491,6 +654,8 @@ hipError_t launch_skinny(const Gemm<T> &g, int splitk, T *work, hipStream_t s)
     return launch_mt<T, 4>(a, s_kc, l_kc, trans_out, s);
 }
 
+template bool skinny_fits<double>(const Gemm<double> &);
+template bool skinny_fits<float>(const Gemm<float> &);
 template int skinny_pick_splitk<double>(const Gemm<double> &, int);
 template int skinny_pick_splitk<float>(const Gemm<float> &, int);
 template hipError_t launch_skinny<double>(const Gemm<double> &, int, double *, hipStream_t);

@@ -43,6 +43,10 @@ hipError_t launch_null(hipStream_t s);
 // the partial sums go to `work` (splitk * M * N elements of T).
 template <typename T>
 hipError_t launch_skinny(const Gemm<T> &g, int splitk, T *work, hipStream_t s);
+// Whether the skinny kernel takes this shape: small side <= 64, and strides
+// small enough that its per-lane load offsets stay below 2^31 bytes.
+template <typename T>
+bool skinny_fits(const Gemm<T> &g);
 // Number of K slices the skinny kernel would use for this shape on this GPU.
 template <typename T>
 int skinny_pick_splitk(const Gemm<T> &g, int num_cus);

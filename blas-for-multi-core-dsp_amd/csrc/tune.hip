@@ -66,8 +66,11 @@ int autotune(bool ta, bool tb, int64_t M, int64_t N, int64_t K, hipStream_t s, T
             max_split = sk > max_split ? sk : max_split;
         }
     }
-    const int64_t small = M < N ? M : N;
-    if (small <= 64) {
+    Gemm<T> shape{};
+    shape.ta = ta; shape.tb = tb;
+    shape.M = M; shape.N = N; shape.K = K;
+    shape.lda = ta ? M : K; shape.ldb = tb ? K : N; shape.ldc = N;
+    if (skinny_fits(shape)) {
         for (int sk = 1; sk <= 64; sk *= 2) {
             if (sk > 1 && K / sk < 256) break;
             cands.push_back(TuneChoice{BLASDSP_ALGO_SKINNY, 0, sk, 0});
