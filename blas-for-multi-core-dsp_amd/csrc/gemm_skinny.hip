@@ -618,19 +618,29 @@ bool skinny_fits(const Gemm<T> &g)
 template <typename T>
 int skinny_pick_splitk(const Gemm<T> &g, int num_cus)
 {
-    // One full round of resident workgroups: a workgroup is one wave per
-    // SIMD, so WPE workgroups fill a CU.
+    // Cost of a launch in units of one unsplit panel's walk over K: s slices
+    // take ceil(wgs * s / slots) rounds of 1/s each (a round that is not full
+    // leaves CUs idle), plus the partial sums, written once per slice and
+    // read once by the reduction, against the streamed K x Nl.  Fewer slices
+    // win near-ties.
     const int64_t ms = g.M <= g.N ? g.M : g.N;
     const int64_t nl = g.M <= g.N ? g.N : g.M;
     const int wpe = ms <= 16 ? SkinnyOcc<T, 1>::WPE : (ms <= 32 ? SkinnyOcc<T, 2>::WPE : SkinnyOcc<T, 4>::WPE);
     const int64_t wgs = (nl + kWaves * kPanel - 1) / (kWaves * kPanel);
-    const int64_t want = (int64_t)wpe * (num_cus > 0 ? num_cus : 256);
-    int64_t sk = (want + wgs - 1) / wgs;
-    const int64_t max_by_k = g.K / 512 > 0 ? g.K / 512 : 1;  // slices of >= 512
-    if (sk > max_by_k) sk = max_by_k;
-    if (sk > 64) sk = 64;
-    if (sk < 1) sk = 1;
-    return (int)sk;
+    const int64_t slots = (int64_t)wpe * (num_cus > 0 ? num_cus : 256);
+    int64_t max_s = g.K / 512;  // slices of >= 512
+    if (max_s > 64) max_s = 64;
+    int best = 1;
+    double best_cost = (double)((wgs + slots - 1) / slots);
+    for (int64_t sk = 2; sk <= max_s; ++sk) {
+        const double rounds = (double)((wgs * sk + slots - 1) / slots);
+        const double cost = rounds / (double)sk + (double)(2 * sk + 1) * (double)ms / (double)g.K;
+        if (cost < 0.97 * best_cost) {
+            best = (int)sk;
+            best_cost = cost;
+        }
+    }
+    return best;
 }
 
 template <typename T>
