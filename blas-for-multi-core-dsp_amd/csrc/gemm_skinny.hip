@@ -91,15 +91,29 @@ struct SkinnyArgs {
 constexpr int kWaves = 4;      // waves per workgroup, each an independent panel
 constexpr int kPanel = 64;     // O columns per wave
 
-// K per block: one 128-byte line of a k-contiguous row (32 FP32 / 16 FP64).
+// 128-byte lines of each K-contiguous row of L per block.  A wave's block
+// covers 64 rows, and with one line per row the concurrent waves touch 128
+// bytes of many distinct rows at a time.  Two lines measured 1-3 % faster for
+// FP64 (16 and 32 rows of S) and 0.5-1.4 % slower for FP32 at K = 8192;
+// 64-row S keeps one line (the larger block does not fit its registers).
+constexpr int kLinesKC = 2;
 template <typename T>
+__host__ __device__ constexpr int lines_of(bool l_kc, int mt)
+{
+    return l_kc && mt <= 2 && sizeof(T) == 8 ? kLinesKC : 1;
+}
+
+// K per block: LINES 128-byte lines (32 FP32 / 16 FP64 per line).
+template <typename T, int LINES>
 struct Blk {
     static constexpr int V = SK<T>::V;            // elements per 16-byte vector
-    static constexpr int KB = 128 / (int)sizeof(T);
-    static constexpr int RUN = KB / 4;            // k-run per lane group (8 / 4)
+    static constexpr int KB = LINES * 128 / (int)sizeof(T);
+    static constexpr int RUN = KB / 4;            // k-run per lane group
     static constexpr int NSTEP = RUN;             // MFMA k-steps per block
-    static constexpr int RV = RUN / V;            // vectors per run (2)
-    static constexpr int NLV = 4 * RV;            // L vectors per lane per block (8)
+    static constexpr int RV = RUN / V;            // vectors per run
+    static constexpr int NLV = 4 * RV;            // L vectors per lane per block
+    static constexpr int CPR = KB / V;            // 16-byte chunks per K-contiguous row
+    static constexpr int RPI = 64 / CPR;          // rows per load instruction (K-contiguous)
 };
 
 // Column of O held by lane column c of MFMA column tile t.
@@ -120,7 +134,7 @@ __device__ __forceinline__ int col_of(int t, int c)
 template <typename T, bool L_KC>
 __device__ __forceinline__ int64_t k_of(int64_t k0, int g, int j)
 {
-    constexpr int V = Blk<T>::V, RUN = Blk<T>::RUN;
+    constexpr int V = Blk<T, 1>::V, RUN = Blk<T, 1>::RUN;
     if (L_KC) return k0 + (j / V) * 4 * V + g * V + (j % V);
     return k0 + g * RUN + j;
 }
@@ -130,7 +144,7 @@ __device__ __forceinline__ int64_t k_of(int64_t k0, int g, int j)
 template <typename T, int MT, bool S_KC, bool L_KC>
 struct Block {
     using vec_t = typename SK<T>::vec_t;
-    using B_ = Blk<T>;
+    using B_ = Blk<T, lines_of<T>(L_KC, MT)>;
     static constexpr int V = B_::V;
     vec_t l[B_::NLV];
     // S: k-contiguous -> runs as vectors; m-contiguous -> scalars per step
@@ -179,22 +193,23 @@ struct LaneOff {
 template <typename T, bool L_KC>
 __device__ __forceinline__ int kgrp(int g)
 {
-    return L_KC ? g * Blk<T>::V : g * Blk<T>::RUN;
+    return L_KC ? g * Blk<T, 1>::V : g * Blk<T, 1>::RUN;
 }
 template <typename T, bool L_KC>
 __host__ __device__ constexpr int kstep(int j)
 {
-    return L_KC ? (j / Blk<T>::V) * 4 * Blk<T>::V + j % Blk<T>::V : j;
+    return L_KC ? (j / Blk<T, 1>::V) * 4 * Blk<T, 1>::V + j % Blk<T, 1>::V : j;
 }
 
 template <typename T, int MT, int SM, bool L_KC>
 __device__ __forceinline__ LaneOff<MT> lane_offsets(const SkinnyArgs<T> &a, int c, int g)
 {
-    constexpr int V = Blk<T>::V, RUN = Blk<T>::RUN;
+    using B_ = Blk<T, lines_of<T>(L_KC, MT)>;
+    constexpr int V = B_::V, RUN = B_::RUN;
     constexpr int64_t s = sizeof(T);
     const int lane = g * 16 + c;
     LaneOff<MT> o;
-    if (L_KC) o.l = (uint32_t)(((int64_t)(lane >> 3) * a.ldl + (lane & 7) * V) * s);
+    if (L_KC) o.l = (uint32_t)(((int64_t)(lane / B_::CPR) * a.ldl + (lane % B_::CPR) * V) * s);
     else o.l = (uint32_t)(((int64_t)g * RUN * a.ldl + V * c) * s);
     const int64_t kg = kgrp<T, L_KC>(g);
 #pragma unroll
@@ -239,14 +254,15 @@ template <typename T, int MT, int SM, bool L_KC>
 __device__ __forceinline__ void load_full(Block<T, MT, SM != 0, L_KC> &b, const SkinnyArgs<T> &a,
                                           const LaneOff<MT> &o, int64_t n0, int64_t k0)
 {
-    using B_ = Blk<T>;
+    using B_ = Blk<T, lines_of<T>(L_KC, MT)>;
     constexpr int V = B_::V;
     if (L_KC) {
-        // vector i: 16 bytes of row n0 + 8 i + lane / 8, chunk lane % 8; each
-        // instruction reads whole 128-byte lines of 8 rows, which
+        // vector i: 16 bytes of row n0 + RPI i + lane / CPR, chunk lane % CPR;
+        // each instruction reads whole KB-long row pieces of RPI rows, which
         // stage_block() turns into MFMA operands through LDS
 #pragma unroll
-        for (int i = 0; i < B_::NLV; ++i) b.l[i] = bload_vec<T, kAuxNT>(a.L + (n0 + 8 * i) * a.ldl + k0, o.l);
+        for (int i = 0; i < B_::NLV; ++i)
+            b.l[i] = bload_vec<T, kAuxNT>(a.L + (n0 + B_::RPI * i) * a.ldl + k0, o.l);
     } else {
         // step j: row k_of(k0, g, j), columns n0 + h*16V + V*c .. + V-1
 #pragma unroll
@@ -286,7 +302,7 @@ __device__ __forceinline__ void load_tail(Block<T, MT, SM != 0, L_KC> &b, const 
                                           int64_t n0, int64_t k0, int64_t kend, int c, int g)
 {
     using vec_t = typename SK<T>::vec_t;
-    using B_ = Blk<T>;
+    using B_ = Blk<T, lines_of<T>(L_KC, MT)>;
     constexpr int V = B_::V;
     // ---- L ----
     if (L_KC) {
@@ -356,11 +372,12 @@ __device__ __forceinline__ void load_tail(Block<T, MT, SM != 0, L_KC> &b, const 
     }
 }
 
-// LDS row stride of the staging buffer: one 128-byte K block plus 16 bytes,
-// so the 16-byte operand reads of 8 consecutive lanes hit distinct banks.
-template <typename T>
+// LDS row stride of the staging buffer: one K-contiguous block row plus 16
+// bytes, so the 16-byte operand reads of 8 consecutive lanes hit distinct
+// banks.
+template <typename T, int MT>
 struct Stage {
-    static constexpr int ROW = Blk<T>::KB + 16 / (int)sizeof(T);
+    static constexpr int ROW = Blk<T, lines_of<T>(true, MT)>::KB + 16 / (int)sizeof(T);
     static constexpr int ELEMS = kPanel * ROW;   // per wave
 };
 
@@ -373,12 +390,12 @@ template <typename T, int MT, bool S_KC>
 __device__ __forceinline__ void stage_block(Block<T, MT, S_KC, true> &b, T *w, int c, int g)
 {
     using vec_t = typename SK<T>::vec_t;
-    using B_ = Blk<T>;
-    constexpr int V = B_::V, ROW = Stage<T>::ROW;
+    using B_ = Blk<T, lines_of<T>(true, MT)>;
+    constexpr int V = B_::V, ROW = Stage<T, MT>::ROW;
     const int lane = g * 16 + c;
 #pragma unroll
     for (int i = 0; i < B_::NLV; ++i)
-        *reinterpret_cast<vec_t *>(w + (8 * i + (lane >> 3)) * ROW + (lane & 7) * V) = b.l[i];
+        *reinterpret_cast<vec_t *>(w + (B_::RPI * i + lane / B_::CPR) * ROW + (lane % B_::CPR) * V) = b.l[i];
     __builtin_amdgcn_wave_barrier();
 #pragma unroll
     for (int t = 0; t < 4; ++t)
@@ -393,7 +410,7 @@ __device__ __forceinline__ void mma_block(typename SK<T>::acc_t (&acc)[MT][4],
                                           const Block<T, MT, S_KC, L_KC> &b)
 {
 #pragma unroll
-    for (int j = 0; j < Blk<T>::NSTEP; ++j)
+    for (int j = 0; j < Blk<T, lines_of<T>(L_KC, MT)>::NSTEP; ++j)
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
@@ -424,7 +441,11 @@ __device__ __forceinline__ void consume_block(typename SK<T>::acc_t (&acc)[MT][4
 template <typename T, int MT>
 struct SkinnyOcc {
     static constexpr int WPE = MT <= 2 ? 1 : (sizeof(T) == 8 ? 1 : 2);
-    static constexpr int DEPTH = MT == 1 ? 4 : (MT == 2 ? 3 : 1);
+    // FP64 blocks of K-contiguous L are kLinesKC times larger
+    static constexpr int depth(bool l_kc)
+    {
+        return lines_of<T>(l_kc, MT) == 2 ? 2 : (MT == 1 ? 4 : (MT == 2 ? 3 : 1));
+    }
 };
 
 template <typename T, int MT, int SM, bool L_KC, bool TRANS_OUT>
@@ -450,20 +471,20 @@ skinny_kernel(SkinnyArgs<T> a)
 #pragma unroll
         for (int t = 0; t < 4; ++t) acc[i][t] = acc_t{0, 0, 0, 0};
 
-    constexpr int KB = Blk<T>::KB;
+    constexpr int KB = Blk<T, lines_of<T>(L_KC, MT)>::KB;
     const bool panel_full = a.l_vec && (n0 + kPanel <= a.Nl);
     const int64_t kfull = panel_full ? kb + ((ke - kb) / KB) * KB : kb;
 
     // K-contiguous L is staged through the wave's own LDS rows.
-    __shared__ __attribute__((aligned(16))) T stage[L_KC ? kWaves * Stage<T>::ELEMS : 1];
-    T *const w = stage + (L_KC ? wave * Stage<T>::ELEMS : 0);
+    __shared__ __attribute__((aligned(16))) T stage[L_KC ? kWaves * Stage<T, MT>::ELEMS : 1];
+    T *const w = stage + (L_KC ? wave * Stage<T, MT>::ELEMS : 0);
 
     // Full blocks: unmasked loads into a ring of D + 1 register blocks.
     // Block j lives in b[j % (D + 1)]; while block j is multiplied, the loads
     // of blocks j + 1 .. j + D are in flight.  The steady-state loop runs
     // whole rounds of D + 1 blocks, so every ring index is a compile-time
     // constant and no block is copied between registers.
-    constexpr int D = SkinnyOcc<T, MT>::DEPTH;
+    constexpr int D = SkinnyOcc<T, MT>::depth(L_KC);
     const int64_t nb = (kfull - kb) / KB;
     if (nb > 0) {
         const LaneOff<MT> o = lane_offsets<T, MT, SM, L_KC>(a, c, g);
@@ -607,9 +628,11 @@ bool skinny_fits(const Gemm<T> &g)
     roles(g, a, s_kc, l_kc, trans_out);
     if (a.Ms > 64 || a.Ms <= 0 || a.Nl <= 0) return false;
     // largest lane offsets of lane_offsets(), in bytes (rows of S up to 63)
-    constexpr double V = SK<T>::V, RUN = Blk<T>::RUN, s = sizeof(T);
+    constexpr double V = SK<T>::V, RUN = Blk<T, 1>::RUN, s = sizeof(T);
+    const int lines = lines_of<T>(l_kc, a.Ms <= 16 ? 1 : (a.Ms <= 32 ? 2 : 4));
+    const double CPR = lines * 128 / s / V, RPI = 64 / CPR;
     const double kg = 3 * (l_kc ? V : RUN);
-    const double lmax = l_kc ? (7.0 * (double)a.ldl + 7 * V) * s : (3 * RUN * (double)a.ldl + 15 * V) * s;
+    const double lmax = l_kc ? ((RPI - 1) * (double)a.ldl + (CPR - 1) * V) * s : (3 * RUN * (double)a.ldl + 15 * V) * s;
     const double smax = s_kc ? (63.0 * (double)a.lds + kg) * s : (kg * (double)a.lds + 63) * s;
     // a 16-byte access at the offset stays inside the buffer resource's 2^31 - 1 bytes
     return lmax + 16 < 2147483647.0 && smax + 16 < 2147483647.0;
@@ -651,7 +674,7 @@ hipError_t launch_skinny(const Gemm<T> &g, int splitk, T *work, hipStream_t s)
     bool s_kc, l_kc, trans_out;
     roles(g, a, s_kc, l_kc, trans_out);
     if (splitk < 1) splitk = 1;
-    constexpr int KB = Blk<T>::KB;
+    const int KB = lines_of<T>(l_kc, a.Ms <= 16 ? 1 : (a.Ms <= 32 ? 2 : 4)) * Blk<T, 1>::KB;
     int64_t kslice = (g.K + splitk - 1) / splitk;
     kslice = (kslice + KB - 1) / KB * KB;
     const int64_t nsplit = (g.K + kslice - 1) / kslice;
