@@ -16,28 +16,30 @@
 // index maps instead of executed.
 //
 // Each wave owns a 64-column panel of O (all Ms rows, MT = Ms/16 MFMA row
-// tiles x 4 column tiles) and walks K in blocks of one 128-byte line (32
-// FP32 or 16 FP64 values).  L is read straight
-// from HBM into MFMA operand registers with 16-byte vector loads and no LDS
-// round trip: every element of L is loaded exactly once.  The MFMA K index is
-// permuted so that each lane's vector holds whole operands:
+// tiles x 4 column tiles) and walks K in blocks of one 128-byte line per row
+// (32 FP32 or 16 FP64 values; two lines for K-contiguous FP64 L).  Every
+// element of L is loaded exactly once, with non-temporal 16-byte loads.  The
+// MFMA K index is permuted so that each lane's vector holds whole operands:
 //   * L stored k-major ([k][nl], nl contiguous): lane (c, g) = (l%16, l/16)
 //     loads row k0 + g*R + j, columns n0 + V*c .. + V-1 at virtual step j
-//     (R = block / 4); element e of the vector is the B operand of column
-//     tile e, which therefore covers columns n0 + V*c + e (an interleaved
-//     column map undone at the store);
-//   * L stored nl-major ([nl][k], k contiguous): lane (c, g) loads V
-//     consecutive k of row n0+16t+c per vector, the four lane groups side by
-//     side, so each load instruction reads 64 contiguous bytes of 16 rows
-//     and two of them the whole 128-byte line.
+//     (R = block / 4) straight into operand registers; element e of the
+//     vector is the B operand of column tile e, which therefore covers
+//     columns n0 + V*c + e (an interleaved column map undone at the store);
+//   * L stored nl-major ([nl][k], k contiguous): each load instruction reads
+//     whole row pieces of the block, which the wave writes to its own LDS
+//     rows and reads back as operands (row n0+16t+c, V consecutive k per
+//     vector, the four lane groups side by side).
 // S is read with the same k pairing as L (k_of below; a run along k when S
-// is k-contiguous).
-// S is tiny (<= 64 x K) and is served from L1/L2.
+// is k-contiguous); it is tiny (<= 64 x K) and served from L1/L2.
 //
-// When the Nl panels alone do not give >= 4 workgroups per CU, K is split
-// into slices (grid.y); each slice writes its partial O to a workspace and a
-// second kernel sums the slices in fixed order (deterministic, no atomics) and
-// applies C = alpha * sum + beta * C.
+// Full blocks are buffer loads from wave-uniform bases with no lane
+// branches, kept in flight in a ring of register blocks (load_full and the
+// ring in skinny_kernel); ragged blocks take bounds-checked loads.
+//
+// When the Nl panels alone leave the GPU's workgroup slots poorly filled, K
+// is split into slices (grid.y, skinny_pick_splitk); each slice writes its
+// partial O to a workspace and a second kernel sums the slices in fixed order
+// (deterministic, no atomics) and applies C = alpha * sum + beta * C.
 #include "internal.h"
 
 namespace blasdsp {
@@ -167,10 +169,12 @@ struct Block {
 // The streamed operand is read exactly once: non-temporal loads keep it from
 // displacing the small operand in the caches (a plain 4 GiB read streams at
 // 6.1-6.3 TB/s on MI355X, the same read non-temporal at 6.96-6.98 TB/s).
-// Used where each load instruction covers whole 64-byte runs of a row
-// (L stored with the long side contiguous); where one instruction covers
-// half of each of 16 rows' 128-byte lines (L stored K-contiguous), measured
-// non-temporal loads ran 10-15 % slower than plain ones, so those stay plain.
+// Full blocks read whole row pieces and always use them (kAuxNT below).  In
+// ragged blocks they are used where each instruction covers whole 64-byte
+// runs of a row (L stored with the long side contiguous); in the operand
+// layout of K-contiguous L an instruction covers half of each of 16 rows'
+// 128-byte lines, where non-temporal loads measured 10-15 % slower than
+// plain ones, so those stay plain.
 template <typename V>
 __device__ __forceinline__ V stream_load(const V *p)
 {
