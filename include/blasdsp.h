@@ -158,10 +158,13 @@ enum blasdsp_algo {
     BLASDSP_ALGO_TILE = 1,   /* MFMA tile kernel                               */
     BLASDSP_ALGO_SKINNY = 2  /* bandwidth-bound skinny kernel (+ split-K)      */
 };
-/* splitk: K slices, their partial products added in a fixed order by a
+/* BLASDSP_ALGO_SKINNY takes shapes whose small side is <= 64 and whose
+ * leading dimensions keep the kernel's per-lane load offsets below 2^31
+ * bytes (about 20 million elements); other shapes run on the tile kernel.
+ * splitk: K slices, their partial products added in a fixed order by a
  * second kernel (deterministic).  <= 0: chosen by the library — the tile
  * kernel splits K only when its C tiles fill the GPU's workgroup slots
- * poorly, the skinny kernel when its long side does not fill the GPU. */
+ * poorly, the skinny kernel when its long side leaves them poorly filled. */
 BLASDSP_API int blasdsp_dgemm_ex(char transA, char transB, int64_t M, int64_t N, int64_t K,
                      double alpha, const double *A, int64_t lda,
                      const double *B, int64_t ldb, double beta,
