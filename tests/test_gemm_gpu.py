@@ -286,6 +286,35 @@ def test_skinny_full_size(orc, blas, dev, prec, kind):
     assert serr <= tol, serr
 
 
+@pytest.mark.parametrize("ldb", [20_000_000, 23_000_000])
+def test_skinny_wide_leading_dimension(orc, blas, dev, ldb):
+    """shortA FP32 (M = 8, K = 32, N = 96) with B a 96-column window of rows
+    ldb elements apart, through ALGO_SKINNY.  Below about 22.4 million
+    elements the skinny kernel's per-lane byte offsets (up to 96 rows of B)
+    stay under 2^31 and it runs the GEMM; above, the call falls back to the
+    tile kernel.  Both against the oracle."""
+    import torch
+
+    M, N, K = 8, 96, 32
+    A = torch.empty((M, K), dtype=torch.float32, device=dev)
+    Bbuf = torch.empty((K, ldb), dtype=torch.float32, device=dev)
+    B = Bbuf[:, :N]
+    C = torch.empty((M, N), dtype=torch.float32, device=dev)
+    blas.fill(A, M, K, 0, 31)
+    blas.fill(B, K, N, 0, 32)
+    blas.fill(C, M, N, 0, 33)
+    c0 = np.ascontiguousarray(C.cpu().numpy())
+    blas.gemm("N", "N", A, B, C, 1.5, -0.5, algo=blas.ALGO_SKINNY)
+    torch.cuda.synchronize()
+    a = np.ascontiguousarray(A.cpu().numpy())
+    b = np.ascontiguousarray(B.cpu().numpy())
+    ref = orc.gemm_xp("N", "N", a, b, c0, 1.5, -0.5)
+    err, _ = orc.normwise(np.ascontiguousarray(C.cpu().numpy()), ref)
+    del Bbuf
+    torch.cuda.empty_cache()
+    assert err <= tol32(K), err
+
+
 def test_stream_release_frees_scratch(orc, blas, dev):
     """A split-K GEMM and a grouped call give a stream scratch memory and a
     cached plan; blasdsp_stream_release frees them (twice is harmless), and
