@@ -66,7 +66,7 @@ struct PerCall {
 template <typename T, int BM, bool TA, bool TB>
 __global__ void __launch_bounds__((PerCall<T, BM>::Cf::THREADS))
 __attribute__((amdgpu_waves_per_eu(PerCall<T, BM>::Cf::WPE, PerCall<T, BM>::Cf::WPE)))
-tile_gemm_kernel(Gemm<T> g, int tiles_m, int tiles_n, bool a_vec, bool b_vec, int64_t kslice, T *W)
+tile_gemm_kernel(Gemm<T> g, int tiles_m, int tiles_n, bool a_vec, bool b_vec, bool full_ok, int64_t kslice, T *W)
 {
     using Cf = typename PerCall<T, BM>::Cf;
     __shared__ __attribute__((aligned(16))) T lds[Cf::LDS_ELEMS];
@@ -76,7 +76,7 @@ tile_gemm_kernel(Gemm<T> g, int tiles_m, int tiles_n, bool a_vec, bool b_vec, in
     // split-K: slice blockIdx.y of K writes its partial to W (one M x N per slice)
     const int64_t kb = (int64_t)blockIdx.y * kslice, ke = min(g.K, kb + kslice);
     T *w = W ? W + (int64_t)blockIdx.y * g.M * g.N : nullptr;
-    if (tile::full_tile<Cf>(g, m0, n0, kb, ke, a_vec, b_vec))
+    if (full_ok && tile::full_tile<Cf>(g, m0, n0, kb, ke, a_vec, b_vec))
         tile::tile_body<T, BM, TA, TB, Cf::NW, true>(g, m0, n0, a_vec, b_vec, lds, kb, ke, w);
     else
         tile::tile_body<T, BM, TA, TB, Cf::NW, false>(g, m0, n0, a_vec, b_vec, lds, kb, ke, w);
@@ -139,16 +139,22 @@ hipError_t launch_tile_bm(const Gemm<T> &g, int splitk, T *work, hipStream_t s)
     T *W = slices > 1 ? work : nullptr;
     const bool a_vec = tile::vec_ok(g.A, g.lda);
     const bool b_vec = tile::vec_ok(g.B, g.ldb);
+    // the unchecked body addresses its loads by 32-bit lane offsets
+    const bool full_ok = tile::full_offsets_fit<T, BM>(g.lda) && tile::full_offsets_fit<T, BM>(g.ldb);
     const dim3 grid((unsigned)nwg, (unsigned)slices), block(PerCall<T, BM>::Cf::THREADS);
     const int tmi = (int)tiles_m, tni = (int)tiles_n;
     if (!g.ta && !g.tb)
-        hipLaunchKernelGGL((tile_gemm_kernel<T, BM, false, false>), grid, block, 0, s, g, tmi, tni, a_vec, b_vec, kslice, W);
+        hipLaunchKernelGGL((tile_gemm_kernel<T, BM, false, false>), grid, block, 0, s, g, tmi, tni, a_vec, b_vec,
+                           full_ok, kslice, W);
     else if (!g.ta && g.tb)
-        hipLaunchKernelGGL((tile_gemm_kernel<T, BM, false, true>), grid, block, 0, s, g, tmi, tni, a_vec, b_vec, kslice, W);
+        hipLaunchKernelGGL((tile_gemm_kernel<T, BM, false, true>), grid, block, 0, s, g, tmi, tni, a_vec, b_vec,
+                           full_ok, kslice, W);
     else if (g.ta && !g.tb)
-        hipLaunchKernelGGL((tile_gemm_kernel<T, BM, true, false>), grid, block, 0, s, g, tmi, tni, a_vec, b_vec, kslice, W);
+        hipLaunchKernelGGL((tile_gemm_kernel<T, BM, true, false>), grid, block, 0, s, g, tmi, tni, a_vec, b_vec,
+                           full_ok, kslice, W);
     else
-        hipLaunchKernelGGL((tile_gemm_kernel<T, BM, true, true>), grid, block, 0, s, g, tmi, tni, a_vec, b_vec, kslice, W);
+        hipLaunchKernelGGL((tile_gemm_kernel<T, BM, true, true>), grid, block, 0, s, g, tmi, tni, a_vec, b_vec,
+                           full_ok, kslice, W);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || slices == 1) return e;
     const int64_t total = g.M * g.N;

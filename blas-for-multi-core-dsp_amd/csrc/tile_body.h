@@ -97,12 +97,29 @@ struct Stager {
 
     // X: stored matrix with R rows, Cn columns, leading dimension ld; the tile
     // starts at stored (r0, c0).  Out-of-range elements read as zero.  FULL:
-    // the caller guarantees the whole tile is in range and vector-aligned.
+    // the caller guarantees the whole tile is in range, vector-aligned and
+    // within full_offsets_fit(); the loads are then buffer loads from the tile's
+    // workgroup-uniform origin (a scalar resource) plus one 32-bit lane
+    // offset, which keeps 64-bit per-lane addresses out of the registers.
     template <bool FULL>
     __device__ __forceinline__ void load(const T *__restrict__ X, int64_t ld, int64_t R, int64_t Cn,
                                          int64_t r0, int64_t c0, bool vec, int tid)
     {
-        const bool full = FULL || (vec && (r0 + TR <= R) && (c0 + TC <= Cn));
+        if (FULL) {
+            // vector i: row tid / CPR + i * (THREADS / CPR), column (tid % CPR) * V
+            constexpr int RSTEP = Cf::THREADS / CPR;
+            static_assert(Cf::THREADS % CPR == 0, "whole rows per step");
+            const uint32_t off = (uint32_t)(((int64_t)(tid / CPR) * ld + (tid % CPR) * Cf::V) * (int64_t)sizeof(T));
+#pragma unroll
+            for (int i = 0; i < NCH; ++i) {
+                const T *base = X + (r0 + (int64_t)i * RSTEP) * ld + c0;
+                const __amdgpu_buffer_rsrc_t rs =
+                    __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(base), (short)0, 0x7fffffff, 0x00020000);
+                r[i] = __builtin_bit_cast(vec_t, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+            }
+            return;
+        }
+        const bool full = vec && (r0 + TR <= R) && (c0 + TC <= Cn);
         if (full) {
 #pragma unroll
             for (int i = 0; i < NCH; ++i) {
@@ -137,6 +154,14 @@ struct Stager {
         }
     }
 };
+
+// The FULL body's lane offsets (up to BM rows of a stored operand tile) stay
+// below 2^31 bytes for leading dimension ld.
+template <typename T, int BM>
+inline bool full_offsets_fit(int64_t ld)
+{
+    return (double)(BM - 1) * (double)ld * sizeof(T) + 4096.0 < 2147483647.0;
+}
 
 // True when tile (m0, n0) over K range [kb, ke) can take the FULL body: both
 // operands vector-aligned, the tile inside C and the range a whole number of
