@@ -90,19 +90,35 @@ struct SkinnyArgs {
     bool l_vec, s_vec;
 };
 
-constexpr int kWaves = 4;      // waves per workgroup, each an independent panel
+constexpr int kWaves = 4;      // panels per workgroup, 64 columns of O each
 constexpr int kPanel = 64;     // O columns per wave
+
+// Waves that share one panel inside a workgroup, each walking its own part
+// of the K range (1 or 2).  With 2, a workgroup is 8 waves, two per SIMD on
+// the same four panels: while one wave of a SIMD waits for its loads the
+// other issues MFMAs, and the two partial sums meet in LDS at the end, so
+// the streamed operand is still read exactly once.  Measured on MI355X at
+// K = 8192, long side 65536: 32 rows of S (where the matrix cores must run
+// at ~65 % of peak to keep pace with the stream) gain 1-16 % per case, except
+// FP64 with k-major L, which is 1-3 % slower and keeps one wave; 8 and 16
+// rows gain nothing.
+template <typename T>
+__host__ __device__ constexpr int skinny_ks(int mt, bool l_kc)
+{
+    return mt == 2 && (sizeof(T) == 4 || l_kc) ? 2 : 1;
+}
 
 // 128-byte lines of each K-contiguous row of L per block.  A wave's block
 // covers 64 rows, and with one line per row the concurrent waves touch 128
 // bytes of many distinct rows at a time.  Two lines measured 1-3 % faster for
-// FP64 (16 and 32 rows of S) and 0.5-1.4 % slower for FP32 at K = 8192;
-// 64-row S keeps one line (the larger block does not fit its registers).
+// FP64 (16 and 32 rows of S, one wave per SIMD) and 0.5-1.4 % slower for FP32
+// at K = 8192; 64-row S keeps one line (the larger block does not fit its
+// registers), and so do K-split waves (two waves share a SIMD's registers).
 constexpr int kLinesKC = 2;
 template <typename T>
 __host__ __device__ constexpr int lines_of(bool l_kc, int mt)
 {
-    return l_kc && mt <= 2 && sizeof(T) == 8 ? kLinesKC : 1;
+    return l_kc && mt <= 2 && sizeof(T) == 8 && skinny_ks<T>(mt, l_kc) == 1 ? kLinesKC : 1;
 }
 
 // K per block: LINES 128-byte lines (32 FP32 / 16 FP64 per line).
@@ -442,32 +458,58 @@ __device__ __forceinline__ void consume_block(typename SK<T>::acc_t (&acc)[MT][4
 // than 2-4 waves with rings of 1-3, and ring depths 3-8 are alike.  One wave
 // per SIMD also needs no K split at that size.  64-row FP32 keeps 2 waves, 1
 // block.
-template <typename T, int MT>
+// With K-split waves (KS = 2) each wave has half the registers, so the rings
+// are shorter: 8 waves x 3 (16 rows) or x 2 (32 rows) blocks per CU.
+template <typename T, int MT, bool L_KC>
 struct SkinnyOcc {
-    static constexpr int WPE = MT <= 2 ? 1 : (sizeof(T) == 8 ? 1 : 2);
+    static constexpr int KS = skinny_ks<T>(MT, L_KC);
+    static constexpr int WPE = KS * (MT <= 2 ? 1 : (sizeof(T) == 8 ? 1 : 2));
     // FP64 blocks of K-contiguous L are kLinesKC times larger
-    static constexpr int depth(bool l_kc)
-    {
-        return lines_of<T>(l_kc, MT) == 2 ? 2 : (MT == 1 ? 4 : (MT == 2 ? 3 : 1));
-    }
+    static constexpr int D = KS == 2 ? (MT == 1 ? 3 : 2)
+                                     : (lines_of<T>(L_KC, MT) == 2 ? 2 : (MT == 1 ? 4 : (MT == 2 ? 3 : 1)));
 };
 
+// LDS of one workgroup: the staging rows of K-contiguous L (one set per wave)
+// and, with K-split waves, the partial sums of the second half (MT x 4 tiles
+// x 4 registers x 64 lanes per panel), which reuse the staging rows.
+template <typename T, int MT, bool L_KC>
+constexpr int skinny_lds_elems()
+{
+    constexpr int KS = skinny_ks<T>(MT, L_KC);
+    constexpr int st = L_KC ? KS * kWaves * Stage<T, MT>::ELEMS : 0;
+    constexpr int red = KS > 1 ? kWaves * MT * 4 * 4 * 64 : 0;
+    return st > red ? st : (red > 0 ? red : 1);
+}
+
 template <typename T, int MT, int SM, bool L_KC, bool TRANS_OUT>
-__global__ void __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(SkinnyOcc<T, MT>::WPE, SkinnyOcc<T, MT>::WPE)))
+__global__ void __launch_bounds__(256 * skinny_ks<T>(MT, L_KC))
+__attribute__((amdgpu_waves_per_eu(SkinnyOcc<T, MT, L_KC>::WPE, SkinnyOcc<T, MT, L_KC>::WPE)))
 skinny_kernel(SkinnyArgs<T> a)
 {
     constexpr bool S_KC = SM != 0;
+    constexpr int KS = skinny_ks<T>(MT, L_KC);
     using acc_t = typename SK<T>::acc_t;
     // the wave index is made wave-uniform, so n0 and the load bases below
     // live in scalar registers
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int c = lane & 15, g = lane >> 4;
-    const int64_t n0 = ((int64_t)blockIdx.x * kWaves + wave) * kPanel;
-    if (n0 >= a.Nl) return;  // no barriers in this kernel: idle waves may leave
+    const int pw = wave % kWaves, half = wave / kWaves;  // panel; part of K (KS = 2)
+    const int64_t n0 = ((int64_t)blockIdx.x * kWaves + pw) * kPanel;
+    const bool active = n0 < a.Nl;
+    // with KS = 1 there are no barriers in this kernel: idle waves may leave
+    if (KS == 1 && !active) return;
     const int split = blockIdx.y;
-    const int64_t kb = (int64_t)split * a.kslice;
-    const int64_t ke = min(a.K, kb + a.kslice);
+    int64_t kb = (int64_t)split * a.kslice;
+    int64_t ke = min(a.K, kb + a.kslice);
+
+    constexpr int KB = Blk<T, lines_of<T>(L_KC, MT)>::KB;
+    if (KS == 2) {
+        // the first wave of a panel takes the first floor(blocks / 2) K
+        // blocks, the second the rest (with the ragged block, if any)
+        const int64_t kmid = kb + ((ke - kb + KB - 1) / KB / 2) * KB;
+        if (half == 0) ke = kmid;
+        else kb = kmid;
+    }
 
     acc_t acc[MT][4];
 #pragma unroll
@@ -475,51 +517,75 @@ skinny_kernel(SkinnyArgs<T> a)
 #pragma unroll
         for (int t = 0; t < 4; ++t) acc[i][t] = acc_t{0, 0, 0, 0};
 
-    constexpr int KB = Blk<T, lines_of<T>(L_KC, MT)>::KB;
     const bool panel_full = a.l_vec && (n0 + kPanel <= a.Nl);
     const int64_t kfull = panel_full ? kb + ((ke - kb) / KB) * KB : kb;
 
     // K-contiguous L is staged through the wave's own LDS rows.
-    __shared__ __attribute__((aligned(16))) T stage[L_KC ? kWaves * Stage<T, MT>::ELEMS : 1];
+    __shared__ __attribute__((aligned(16))) T stage[skinny_lds_elems<T, MT, L_KC>()];
     T *const w = stage + (L_KC ? wave * Stage<T, MT>::ELEMS : 0);
-
-    // Full blocks: unmasked loads into a ring of D + 1 register blocks.
-    // Block j lives in b[j % (D + 1)]; while block j is multiplied, the loads
-    // of blocks j + 1 .. j + D are in flight.  The steady-state loop runs
-    // whole rounds of D + 1 blocks, so every ring index is a compile-time
-    // constant and no block is copied between registers.
-    constexpr int D = SkinnyOcc<T, MT>::depth(L_KC);
-    const int64_t nb = (kfull - kb) / KB;
-    if (nb > 0) {
-        const LaneOff<MT> o = lane_offsets<T, MT, SM, L_KC>(a, c, g);
-        Block<T, MT, S_KC, L_KC> b[D + 1];
+    if (active) {
+        // Full blocks: unmasked loads into a ring of D + 1 register blocks.
+        // Block j lives in b[j % (D + 1)]; while block j is multiplied, the loads
+        // of blocks j + 1 .. j + D are in flight.  The steady-state loop runs
+        // whole rounds of D + 1 blocks, so every ring index is a compile-time
+        // constant and no block is copied between registers.
+        constexpr int D = SkinnyOcc<T, MT, L_KC>::D;
+        const int64_t nb = (kfull - kb) / KB;
+        if (nb > 0) {
+            const LaneOff<MT> o = lane_offsets<T, MT, SM, L_KC>(a, c, g);
+            Block<T, MT, S_KC, L_KC> b[D + 1];
 #pragma unroll
-        for (int u = 0; u < D; ++u)
-            if (u < nb) load_full<T, MT, SM, L_KC>(b[u], a, o, n0, kb + u * KB);
-        int64_t i = 0;
-        for (; i + 2 * D < nb; i += D + 1) {
+            for (int u = 0; u < D; ++u)
+                if (u < nb) load_full<T, MT, SM, L_KC>(b[u], a, o, n0, kb + u * KB);
+            int64_t i = 0;
+            for (; i + 2 * D < nb; i += D + 1) {
 #pragma unroll
-            for (int u = 0; u <= D; ++u) {
-                load_full<T, MT, SM, L_KC>(b[(u + D) % (D + 1)], a, o, n0, kb + (i + u + D) * KB);
-                consume_block<T, MT, S_KC, L_KC>(acc, b[u], w, c, g);
-            }
-        }
-        // the last nb - i <= 2D blocks; blocks i .. i + D - 1 are loaded
-#pragma unroll
-        for (int u = 0; u < 2 * D; ++u) {
-            if (i + u < nb) {
-                if (i + u + D < nb)
+                for (int u = 0; u <= D; ++u) {
                     load_full<T, MT, SM, L_KC>(b[(u + D) % (D + 1)], a, o, n0, kb + (i + u + D) * KB);
-                consume_block<T, MT, S_KC, L_KC>(acc, b[u % (D + 1)], w, c, g);
+                    consume_block<T, MT, S_KC, L_KC>(acc, b[u], w, c, g);
+                }
+            }
+            // the last nb - i <= 2D blocks; blocks i .. i + D - 1 are loaded
+#pragma unroll
+            for (int u = 0; u < 2 * D; ++u) {
+                if (i + u < nb) {
+                    if (i + u + D < nb)
+                        load_full<T, MT, SM, L_KC>(b[(u + D) % (D + 1)], a, o, n0, kb + (i + u + D) * KB);
+                    consume_block<T, MT, S_KC, L_KC>(acc, b[u % (D + 1)], w, c, g);
+                }
             }
         }
-    }
-    int64_t k0 = kfull;
-    // Ragged tail (partial panel or partial K block): masked loads.
-    for (; k0 < ke; k0 += KB) {
-        Block<T, MT, S_KC, L_KC> b;
-        load_tail<T, MT, SM, L_KC>(b, a, n0, k0, ke, c, g);
-        mma_block<T, MT, S_KC, L_KC>(acc, b);
+        int64_t k0 = kfull;
+        // Ragged tail (partial panel or partial K block): masked loads.
+        for (; k0 < ke; k0 += KB) {
+            Block<T, MT, S_KC, L_KC> b;
+            load_tail<T, MT, SM, L_KC>(b, a, n0, k0, ke, c, g);
+            mma_block<T, MT, S_KC, L_KC>(acc, b);
+        }
+    }  // active
+
+    if constexpr (KS == 2) {
+        // The second wave of each panel hands its sums to the first, which
+        // adds them (first part + second part, a fixed order) and stores.
+        // Every wave of the workgroup reaches both barriers.
+        __syncthreads();  // all staging reads are done: the rows are reused
+        T *red = stage + pw * (MT * 4 * 4 * 64);
+        if (half == 1 && active) {
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) red[((mt * 4 + t) * 4 + r) * 64 + lane] = acc[mt][t][r];
+        }
+        __syncthreads();
+        if (half == 1 || !active) return;
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[mt][t][r] += red[((mt * 4 + t) * 4 + r) * 64 + lane];
     }
 
     // Store: lane (c, g), register r of tile (mt, t) is
@@ -572,7 +638,7 @@ hipError_t launch4(const SkinnyArgs<T> &a, hipStream_t s)
 {
     const int64_t wgs = (a.Nl + kWaves * kPanel - 1) / (kWaves * kPanel);
     hipLaunchKernelGGL((skinny_kernel<T, MT, SM, L_KC, TRANS_OUT>), dim3((unsigned)wgs, a.splitk),
-                       dim3(kWaves * 64), 0, s, a);
+                       dim3(kWaves * 64 * skinny_ks<T>(MT, L_KC)), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || a.splitk == 1) return e;
     const int64_t total = a.Ms * a.Nl;
@@ -652,9 +718,13 @@ int skinny_pick_splitk(const Gemm<T> &g, int num_cus)
     // win near-ties.
     const int64_t ms = g.M <= g.N ? g.M : g.N;
     const int64_t nl = g.M <= g.N ? g.N : g.M;
-    const int wpe = ms <= 16 ? SkinnyOcc<T, 1>::WPE : (ms <= 32 ? SkinnyOcc<T, 2>::WPE : SkinnyOcc<T, 4>::WPE);
+    // resident workgroups per CU: waves per SIMD over the K-split waves per
+    // panel (the same for either layout of L)
+    const int wg_cu = ms <= 16 ? SkinnyOcc<T, 1, false>::WPE / SkinnyOcc<T, 1, false>::KS
+                               : (ms <= 32 ? SkinnyOcc<T, 2, false>::WPE / SkinnyOcc<T, 2, false>::KS
+                                           : SkinnyOcc<T, 4, false>::WPE / SkinnyOcc<T, 4, false>::KS);
     const int64_t wgs = (nl + kWaves * kPanel - 1) / (kWaves * kPanel);
-    const int64_t slots = (int64_t)wpe * (num_cus > 0 ? num_cus : 256);
+    const int64_t slots = (int64_t)wg_cu * (num_cus > 0 ? num_cus : 256);
     int64_t max_s = g.K / 512;  // slices of >= 512
     if (max_s > 64) max_s = 64;
     int best = 1;

@@ -65,7 +65,8 @@ def test_tile_kernel_modes(orc, blas, dev, prec, mode, shape):
 @pytest.mark.parametrize("prec", ["d", "s"])
 @pytest.mark.parametrize("mode", ["NN", "NT", "TN", "TT"])
 @pytest.mark.parametrize("shape", [(8, 300, 1000), (16, 64, 16), (32, 1000, 777), (40, 129, 64),
-                                   (1, 70, 33), (300, 8, 1000), (1000, 32, 777), (129, 40, 64), (70, 1, 33)])
+                                   (1, 70, 33), (300, 8, 1000), (1000, 32, 777), (129, 40, 64), (70, 1, 33),
+                                   (32, 100, 20), (300, 20, 5)])
 def test_skinny_kernel_modes(orc, blas, dev, prec, mode, shape):
     M, N, K = shape
     got, ref = run_case(orc, blas, dev, prec, mode[0], mode[1], M, N, K, 1.5, -0.5, blas.ALGO_SKINNY)
