@@ -51,10 +51,12 @@ ALPHA, BETA = 1.5, -0.5
 MODES = ("NN", "NT", "TN", "TT")
 
 
-def traffic_from_profiles():
-    """HBM bytes per dgemm launch measured with rocprofv3 --pmc (FETCH_SIZE,
-    WRITE_SIZE), if the summary has been recorded under profiles/."""
-    p = ROOT / "profiles" / "dgemm_4096_hbm_traffic.json"
+def traffic_from_profiles(name="dgemm_4096_hbm_traffic.json"):
+    """HBM bytes per launch of a workload's kernel measured with rocprofv3
+    --pmc (FETCH_SIZE, WRITE_SIZE), if the summary has been recorded under
+    profiles/ (dgemm_4096_hbm_traffic.json: the DGEMM tile kernel;
+    skinny_k8192_hbm_traffic.json: the mean over the tall-skinny cases)."""
+    p = ROOT / "profiles" / name
     if not p.exists():
         return None
     try:
@@ -280,7 +282,9 @@ def bench_skinny(args):
                   {"workload": "shortA/thinB, small side 8/16/32, K=8192, long side 65536, NN/NT/TN/TT, "
                                "FP32 and FP64, alpha=beta=1, N(0,1)", "cases": cases},
                   {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                   "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None},
+                   "frac": round(gbs / PEAK_HBM_GBS, 4),
+                   "traffic": traffic_from_profiles("skinny_k8192_hbm_traffic.json"),
+                   "algorithmic_bytes_per_launch": round(tot_bytes / len(cases))},
                   ms_per_step=round(tot_ms / len(cases), 4))
     print(json.dumps(out), flush=True)
 
