@@ -29,6 +29,8 @@ Other workloads (one JSON line each, same keys):
     --workload copy       a plain 2 GiB device-to-device copy (torch): the
                           achievable HBM bandwidth, and the reference kernel
                           that scales rocprofv3 FETCH_SIZE / WRITE_SIZE
+    --workload stencil    five-point stencil sweeps, FP64 haloed grids 2048^2 ..
+                          8192^2: grid points per ms and HBM GB/s
 """
 from __future__ import annotations
 
@@ -55,7 +57,8 @@ def traffic_from_profiles(name="dgemm_4096_hbm_traffic.json"):
     """HBM bytes per launch of a workload's kernel measured with rocprofv3
     --pmc (FETCH_SIZE, WRITE_SIZE), if the summary has been recorded under
     profiles/ (dgemm_4096_hbm_traffic.json: the DGEMM tile kernel;
-    skinny_k8192_hbm_traffic.json: the mean over the tall-skinny cases)."""
+    skinny_k8192_hbm_traffic.json: the mean over the tall-skinny cases;
+    stencil_hbm_traffic.json: the 8192^2 stencil sweep)."""
     p = ROOT / "profiles" / name
     if not p.exists():
         return None
@@ -110,7 +113,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--workload", default="dgemm", choices=["dgemm", "skinny", "precision", "serving", "copy", "grouped",
-                                                        "transpose", "sgemm512", "sweep"])
+                                                        "transpose", "sgemm512", "sweep", "stencil"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1"}))
@@ -118,7 +121,7 @@ def main():
     if args.workload != "dgemm":
         {"skinny": bench_skinny, "precision": bench_precision, "serving": bench_serving,
          "copy": bench_copy, "grouped": bench_grouped, "transpose": bench_transpose,
-         "sgemm512": bench_sgemm512, "sweep": bench_sweep}[args.workload](args)
+         "sgemm512": bench_sgemm512, "sweep": bench_sweep, "stencil": bench_stencil}[args.workload](args)
         return
 
     import torch
@@ -579,6 +582,71 @@ def bench_sweep(args):
                    "dgemm_tflops_by_mode": dg, "irr_thinB_sgemm": tb},
                   {"bound": "mfma", "achieved": round(tf, 3), "peak": 78.6, "unit": "TFLOP/s",
                    "frac": round(tf / 78.6, 4), "traffic": None})
+    print(json.dumps(out), flush=True)
+
+
+def bench_stencil(args):
+    """Five-point stencil sweeps on the reference harness's grids
+    (stencil_2d_V1.2/bin/run.sh: 2048^2 .. 8192^2, FP64, one-element halo),
+    reported as the harness reports them, grid points per ms
+    (matrix_stencil.c:114-117), and as HBM GB/s.  Algorithmic bytes per sweep:
+    the haloed grid read once, the interior written once,
+    8 (M+2)(N+2) + 8 M N.  CPU baseline: the oracle's restatement of the
+    harness's checker on the 8192^2 grid."""
+    import numpy as np
+    import torch
+    import __graft_entry__
+
+    blas = __graft_entry__.load_package()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    reps = max(3, min(args.steps, 20))
+    cases = []
+    tot_bytes = tot_ms = 0.0
+    for n in (2048, 4096, 6144, 8192):
+        A = torch.empty((n + 2, n + 2), dtype=torch.float64, device=dev)
+        blas.fill(A, n + 2, n + 2, blas.UNIFORM, 9)
+        B = torch.empty((n + 2, n + 2), dtype=torch.float64, device=dev)
+        ms = _time_events(lambda: blas.stencil(A, B, halo_out=True), reps, stream)
+        nbytes = 8.0 * (n + 2) ** 2 + 8.0 * n * n
+        cases.append({"M": n, "N": n, "ms": round(ms, 4), "grids_per_ms": round(n * n / ms, 1),
+                      "gbs": round(nbytes / ms / 1e6, 1)})
+        tot_bytes += nbytes
+        tot_ms += ms
+        del A, B
+    big = cases[-1]
+    gbs = tot_bytes / tot_ms / 1e6
+    cpu = None
+    if not args.no_cpu:
+        orc = __graft_entry__.load_oracle()
+        threads = min(16, os.cpu_count() or 1)
+        orc.set_threads(threads)
+        n = 8192
+        Ah = np.random.default_rng(9).random((n + 2, n + 2))
+        Bh = np.empty((n, n))
+        orc.lib().oracle_stencil(n, n, Ah.ctypes.data, n + 2, Bh.ctypes.data, n, 0)  # warm-up
+        sweeps = 4
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            for _ in range(sweeps):
+                orc.lib().oracle_stencil(n, n, Ah.ctypes.data, n + 2, Bh.ctypes.data, n, 0)
+            times.append((time.perf_counter() - t0) / sweeps)
+        tmed = statistics.median(times)
+        cpu = {"value": round(n * n / (tmed * 1e3), 1), "unit": "grid points/ms", "cores": threads,
+               "kind": "port",
+               "sample": f"the oracle's restatement of reference_stencil (matrix_stencil.c:33-47) on the whole "
+                         f"8192^2 grid, gcc -O3 -fopenmp, median of 3 runs of {sweeps} sweeps "
+                         f"({tmed * 1e3:.1f} ms per sweep)"}
+    out = _common(args, big["grids_per_ms"], "grid points/ms", "f64",
+                  {"workload": "2-D five-point stencil sweep, FP64, haloed (M+2) x (N+2) grids, M = N in "
+                               "{2048, 4096, 6144, 8192} (value: 8192^2)", "cases": cases},
+                  {"bound": "hbm", "achieved": round(big["gbs"], 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                   "frac": round(big["gbs"] / PEAK_HBM_GBS, 4),
+                   "traffic": traffic_from_profiles("stencil_hbm_traffic.json"),
+                   "algorithmic_bytes_per_launch": 8 * (8192 + 2) ** 2 + 8 * 8192 ** 2,
+                   "aggregate_gbs_all_sizes": round(gbs, 1)},
+                  cpu, ms_per_step=big["ms"])
     print(json.dumps(out), flush=True)
 
 

@@ -148,6 +148,10 @@ def _declare(L: ctypes.CDLL) -> None:
     L.transpose_A2B.argtypes = [_i64, _i64, _i64, _i64, _vp, _vp, _vp]
     L.blasdsp_transpose_d.argtypes = [_i64, _i64, _vp, _i64, _vp, _i64, _vp]
     L.blasdsp_transpose_s.argtypes = [_i64, _i64, _vp, _i64, _vp, _i64, _vp]
+    for name in ("stencil_vector", "stencil_scalar", "stencil_null"):
+        getattr(L, name).argtypes = [_i64, _i64, _vp, _vp, _vp]
+    L.blasdsp_stencil.argtypes = [_i64, _i64, _vp, _i64, _vp, _i64, ctypes.c_int, _vp]
+    L.blasdsp_stencil_iterate.argtypes = [_i64, _i64, _vp, _vp, _i64, ctypes.c_int, _vp]
     for extra in _EXTRA_DECLS:
         extra(L)
 
@@ -338,6 +342,31 @@ def transpose(A, B, stream=None) -> None:
         raise ValueError("B must be A.shape[::-1] with A's dtype")
     fn = lib().blasdsp_transpose_d if A.dtype == torch.float64 else lib().blasdsp_transpose_s
     check("blasdsp_transpose", fn(M, N, A.data_ptr(), _ld(A), B.data_ptr(), _ld(B), _stream(stream)))
+
+
+def stencil(A, B, halo_out: bool = False, stream=None) -> None:
+    """One five-point stencil sweep of the haloed float64 grid A ((M+2) x
+    (N+2) device tensor, unit column stride) into B: (M+2) x (N+2) with
+    halo_out (interior written), else M x N."""
+    import torch
+
+    M, N = A.shape[0] - 2, A.shape[1] - 2
+    want = (M + 2, N + 2) if halo_out else (M, N)
+    if A.dtype != torch.float64 or B.dtype != torch.float64 or tuple(B.shape) != want:
+        raise ValueError(f"A float64 (M+2) x (N+2), B float64 {want}")
+    check("blasdsp_stencil", lib().blasdsp_stencil(M, N, A.data_ptr(), _ld(A), B.data_ptr(), _ld(B),
+                                                   int(halo_out), _stream(stream)))
+
+
+def stencil_iterate(X, Y, iters: int, stream=None):
+    """`iters` Jacobi sweeps between the haloed grids X and Y (same shape and
+    row stride); returns the tensor holding the last sweep's result."""
+    M, N = X.shape[0] - 2, X.shape[1] - 2
+    if X.shape != Y.shape or _ld(X) != _ld(Y):
+        raise ValueError("X and Y must have the same shape and row stride")
+    check("blasdsp_stencil_iterate", lib().blasdsp_stencil_iterate(M, N, X.data_ptr(), Y.data_ptr(), _ld(X),
+                                                                   iters, _stream(stream)))
+    return Y if iters % 2 else X
 
 
 def fill(X, rows: int, cols: int, dist: int, seed: int, trans: bool = False, pad: float = 0.0, stream=None):

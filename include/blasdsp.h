@@ -337,6 +337,37 @@ BLASDSP_API int blasdsp_transpose_d(int64_t M, int64_t N, const double *A, int64
 BLASDSP_API int blasdsp_transpose_s(int64_t M, int64_t N, const float *A, int64_t lda, float *B,
                                     int64_t ldb, void *stream);
 
+/* ---- 2-D five-point stencil (one Jacobi sweep), FP64 ----------------------
+ * A is an (M+2) x (N+2) grid with a one-element halo.  Every interior point
+ * becomes (top + left + right + down + middle) * 0.2, summed left to right in
+ * double, as the reference's host checker does
+ * (stencil_2d_V1.2/host_code/matrix_stencil.c:33-47).  Out of place only: the
+ * output must not overlap A.  HBM-bound.
+ *   stencil_2d_V1.2/device_code/kernel_stencil.c:1042  stencil_vector(M, N, A, B)
+ *       B is haloed like A, packed (ld N + 2); its interior is written and its
+ *       halo left as it is (the layout of kernel_stencil.c:1095-1096)
+ *   stencil_2d_V1.2/device_code/kernel_stencil.c:14    stencil_scalar(M, N, A, B)
+ *       B is M x N, packed (ld N), as kernel_stencil.c:85-86 and the host checker
+ *   stencil_2d_V1.2/device_code/kernel_stencil.c:1114  stencil_null(M, N, A, B)  (empty launch)
+ * A stream is added. */
+BLASDSP_API int stencil_vector(int64_t M, int64_t N, const double *A, double *B, void *stream);
+BLASDSP_API int stencil_scalar(int64_t M, int64_t N, const double *A, double *B, void *stream);
+BLASDSP_API int stencil_null(int64_t M, int64_t N, const double *A, double *B, void *stream);
+/* Any leading dimensions (lda >= N + 2; ldb >= N + 2 with halo_out, else >= N);
+ * no counterpart in the reference. */
+BLASDSP_API int blasdsp_stencil(int64_t M, int64_t N, const double *A, int64_t lda, double *B, int64_t ldb,
+                                int halo_out, void *stream);
+/* `iters` Jacobi sweeps between two haloed grids X and Y (leading dimension
+ * ld >= N + 2) whose halos both hold the boundary: sweep i reads X and writes
+ * Y when i is even, the other way round when odd, so the last sweep's result
+ * is in Y when iters is odd and in X when it is even.  The reference harness
+ * means this loop (stencil_2d_V1.2/host_code/matrix_stencil.c:106-111), but
+ * its pointer swap (switch_pointer, :28-33) exchanges copies of the pointers
+ * and the launch arguments are fixed before the loop, so there every sweep
+ * reads the same A. */
+BLASDSP_API int blasdsp_stencil_iterate(int64_t M, int64_t N, double *X, double *Y, int64_t ld, int iters,
+                                        void *stream);
+
 /* ---- per-stream resources (no counterpart in the reference) ---------------
  * The library keeps, per (device, stream), a grow-only scratch buffer
  * (split-K partials, split and FP8 operand planes) and the cached plan of

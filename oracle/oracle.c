@@ -530,3 +530,33 @@ void oracle_sgemm_fp8(char transA, char transB, int64_t M, int64_t N, int64_t K,
         }
     free(qa); free(qb); free(ea); free(eb);
 }
+
+/* ------------------------------------------------------------------------- */
+/* 2-D five-point stencil                                                    */
+/* ------------------------------------------------------------------------- */
+
+/* reference_stencil (stencil_2d_V1.2/host_code/matrix_stencil.c:33-47):
+ *   top    = A[row][col + 1]      left  = A[row + 1][col]
+ *   right  = A[row + 1][col + 2]  down  = A[row + 2][col + 1]
+ *   middle = A[row + 1][col + 1]
+ *   out    = (top + left + right + down + middle) * 0.2
+ * in haloed coordinates, the sum taken left to right in double and then
+ * multiplied by the double nearest 0.2.  (The reference's OpenMP loop shares
+ * its five temporaries between threads, a data race; the restatement keeps
+ * them per element, which is what the loop means.)  The reference writes the
+ * packed M x N layout (off = 0, as stencil_scalar does,
+ * stencil_2d_V1.2/device_code/kernel_stencil.c:85-86); stencil_vector writes
+ * the interior of a haloed (M+2) x (N+2) B (off = 1, kernel_stencil.c:1095-1096). */
+void oracle_stencil(int64_t M, int64_t N, const double *A, int64_t lda, double *B, int64_t ldb, int off)
+{
+#pragma omp parallel for num_threads(oracle_get_threads()) schedule(static)
+    for (int64_t row = 0; row < M; row++)
+        for (int64_t col = 0; col < N; col++) {
+            const double top = A[IDX(row, col + 1, lda)];
+            const double left = A[IDX(row + 1, col, lda)];
+            const double right = A[IDX(row + 1, col + 2, lda)];
+            const double down = A[IDX(row + 2, col + 1, lda)];
+            const double middle = A[IDX(row + 1, col + 1, lda)];
+            B[IDX(row + off, col + off, ldb)] = (top + left + right + down + middle) * 0.2;
+        }
+}

@@ -120,6 +120,13 @@ void oracle_sgemm_fp8(char transA, char transB, int64_t M, int64_t N, int64_t K,
                       const float *A, int64_t lda, const float *B, int64_t ldb, double beta,
                       const float *C32, int64_t ldc32, double *C64, int64_t ldc64);
 
+/* 2-D five-point stencil (restates reference_stencil,
+ * stencil_2d_V1.2/host_code/matrix_stencil.c:33-47).  A: (M+2) x (N+2) with a
+ * one-element halo, leading dimension lda >= N+2.  Output (row, col) of the
+ * M x N interior goes to B[(row + off) * ldb + col + off]: off = 0 is the
+ * packed layout, off = 1 the haloed one. */
+void oracle_stencil(int64_t M, int64_t N, const double *A, int64_t lda, double *B, int64_t ldb, int off);
+
 void oracle_set_threads(int n);
 int oracle_get_threads(void);
 

@@ -73,6 +73,7 @@ def lib() -> ctypes.CDLL:
         L.oracle_fp8_block_exp.argtypes = [ctypes.c_float]
         L.oracle_sgemm_fp8.argtypes = [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, ctypes.c_double,
                                        _dp, _i64, _dp, _i64, ctypes.c_double, _dp, _i64, _dp, _i64]
+        L.oracle_stencil.argtypes = [_i64, _i64, _dp, _i64, _dp, _i64, ctypes.c_int]
         L.oracle_set_threads.argtypes = [ctypes.c_int]
         L.oracle_get_threads.restype = ctypes.c_int
         _lib = L
@@ -199,3 +200,17 @@ def normwise(X, R) -> tuple[float, float]:
     fn = lib().oracle_normwise_f64 if X.dtype == np.float64 else lib().oracle_normwise_f32
     err = fn(M, N, _ptr(X), _ld(X), _ptr(R), _ld(R), ctypes.byref(mx))
     return float(err), float(mx.value)
+
+
+def stencil(A: np.ndarray, halo_out: bool = False) -> np.ndarray:
+    """Five-point stencil of a haloed float64 grid A ((M+2) x (N+2) view, rows
+    contiguous): the packed M x N result, or with halo_out the (M+2) x (N+2)
+    grid whose interior holds it (halo copied from A)."""
+    assert A.dtype == np.float64
+    M, N = A.shape[0] - 2, A.shape[1] - 2
+    if halo_out:
+        B = np.array(A, dtype=np.float64, copy=True)
+    else:
+        B = np.empty((M, N), dtype=np.float64)
+    lib().oracle_stencil(M, N, _ptr(A), _ld(A), _ptr(B), _ld(B), 1 if halo_out else 0)
+    return B

@@ -159,3 +159,23 @@ def test_transpose_rejects_bad_arguments(blas):
     assert L.blasdsp_transpose_s(8, 8, a, 8, a + 16, 8, None) == blas.ERR_INVALID        # overlapping
     assert L.transpose_float_32x32_dbuf(0, 3, None, None, None) == blas.SUCCESS          # empty: no launch
     assert L.transpose_A2B(2, 4, 4, 4, a, a + 2048, None) == blas.ERR_INVALID            # lda < N
+
+
+def test_stencil_rejects_bad_arguments(blas):
+    """Checked on the host before any HIP call."""
+    L = blas.lib()
+    buf = ctypes.create_string_buffer(1 << 14)
+    a = ctypes.addressof(buf)
+    b = a + 8192
+    assert L.blasdsp_stencil(-1, 4, a, 6, b, 6, 1, None) == blas.ERR_INVALID
+    assert L.blasdsp_stencil(4, 4, a, 5, b, 6, 1, None) == blas.ERR_INVALID        # lda < N + 2
+    assert L.blasdsp_stencil(4, 4, a, 6, b, 5, 1, None) == blas.ERR_INVALID        # haloed ldb < N + 2
+    assert L.blasdsp_stencil(4, 4, a, 6, b, 3, 0, None) == blas.ERR_INVALID        # packed ldb < N
+    assert L.blasdsp_stencil(4, 4, None, 6, b, 6, 1, None) == blas.ERR_INVALID
+    assert L.blasdsp_stencil(4, 4, a, 6, a + 8 * 20, 6, 1, None) == blas.ERR_INVALID  # overlaps A
+    assert L.blasdsp_stencil(0, 4, None, 6, None, 6, 1, None) == blas.SUCCESS       # empty: no launch
+    assert L.blasdsp_stencil_iterate(4, 4, a, b, 6, -1, None) == blas.ERR_INVALID
+    assert L.blasdsp_stencil_iterate(4, 4, a, b, 6, 0, None) == blas.SUCCESS        # no sweep
+    assert L.stencil_vector(-2, 4, a, b, None) == blas.ERR_INVALID
+    assert L.stencil_scalar(4, 4, a, a + 8, None) == blas.ERR_INVALID               # overlaps A
+    assert L.stencil_null(-1, 4, a, b, None) == blas.ERR_INVALID

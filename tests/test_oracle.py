@@ -382,3 +382,56 @@ def test_fp8_gemm_error_band(orc):
     got = orc.gemm_fp8("N", "N", A, B, C0, 1.0, 0.0)
     err, _ = orc.normwise(got, ref)
     assert 0.005 < err < 0.08, err
+
+
+def _dsp_pairing(A):
+    """The DSP micro-kernel's pairing of the five terms
+    (stencil_2d_V1.2/readme.txt, steps [1]-[18]): rows alternate between
+    right + ((left + top) + (middle + down)) and
+    right + ((left + down) + (top + middle))."""
+    top, left, right = A[:-2, 1:-1], A[1:-1, :-2], A[1:-1, 2:]
+    down, mid = A[2:, 1:-1], A[1:-1, 1:-1]
+    even = right + ((left + top) + (mid + down))
+    odd = right + ((left + down) + (top + mid))
+    out = np.where((np.arange(even.shape[0]) % 2 == 0)[:, None], even, odd)
+    return out * 0.2
+
+
+def test_stencil_matches_host_checker_order(orc):
+    """The oracle restates reference_stencil (matrix_stencil.c:33-47): against
+    numpy's elementwise evaluation of the same left-to-right sum, bit for bit;
+    the DSP kernel's pairing differs in the last bits on random data, so the
+    order the oracle fixes is the checker's."""
+    rng = np.random.default_rng(5)
+    for M, N in ((1, 1), (3, 7), (55, 111), (64, 130)):
+        A = rng.random((M + 2, N + 2))
+        ref = (A[:-2, 1:-1] + A[1:-1, :-2] + A[1:-1, 2:] + A[2:, 1:-1] + A[1:-1, 1:-1]) * 0.2
+        np.testing.assert_array_equal(orc.stencil(A), ref)
+        H = orc.stencil(A, halo_out=True)
+        np.testing.assert_array_equal(H[1:-1, 1:-1], ref)
+        np.testing.assert_array_equal(H[0], A[0])
+        np.testing.assert_array_equal(H[-1], A[-1])
+        np.testing.assert_array_equal(H[:, 0], A[:, 0])
+        np.testing.assert_array_equal(H[:, -1], A[:, -1])
+    A = rng.random((66, 130))
+    assert np.count_nonzero(orc.stencil(A) != _dsp_pairing(A)) > 0
+
+
+def test_stencil_closed_forms(orc):
+    """Constant grid c: every point is c + c + c + c + c (left to right) times
+    0.2.  A linear field a*i + b*j is reproduced (the five-point mean of a
+    linear function is its centre value) to rounding.  A single unit spike
+    spreads 0.2 to itself and its four neighbours only."""
+    for c in (1.0, 0.1, -3.75):
+        A = np.full((9, 12), c)
+        want = (c + c + c + c + c) * 0.2
+        assert np.all(orc.stencil(A) == want)
+    i, j = np.meshgrid(np.arange(40.0), np.arange(50.0), indexing="ij")
+    A = 0.5 * i - 0.25 * j
+    np.testing.assert_allclose(orc.stencil(A), A[1:-1, 1:-1], rtol=0, atol=1e-13)
+    A = np.zeros((9, 9))
+    A[4, 4] = 1.0
+    out = orc.stencil(A)
+    nz = {(int(r), int(c)) for r, c in zip(*np.nonzero(out))}
+    assert nz == {(3, 3), (2, 3), (4, 3), (3, 2), (3, 4)}
+    assert np.all(out[out != 0] == 0.2)
