@@ -10,9 +10,15 @@
 //     16-byte vector loads along whichever dimension is contiguous in HBM, and
 //     the LDS copy keeps that orientation; the MFMA fragment reads adapt to it,
 //     so the transpose modes need no transpose pass and no transposing writes;
-//   * LDS rows are padded (k-contiguous rows by 16 bytes, m/n-contiguous rows
-//     by 16 elements) so that fragment reads are bank-conflict-free per
-//     half-wave;
+//   * LDS rows are padded: m/n-contiguous rows by 16 elements, which makes
+//     their fragment reads bank-conflict-free, and k-contiguous rows by 16
+//     bytes, which leaves a 2-way conflict (rows r and r + 8 of a 16-lane
+//     read share banks; rocprofv3 SQ_LDS_BANK_CONFLICT on MI355X: zero for
+//     two m/n-contiguous operands, one unit per K-contiguous operand).  An
+//     XOR-swizzled k-contiguous layout measured zero conflicts but ran slower
+//     (FP32 8192^3 -8 to -12 %, FP64 4096^3 -1 to -2 %): with four waves per
+//     SIMD the conflicts' latency is hidden, and the swizzle's per-lane
+//     addresses are not;
 //   * two LDS stages: global loads for K tile k+1 are in flight while the
 //     MFMAs consume tile k; one barrier per K tile;
 //   * epilogue C = alpha * acc + beta * C fused into the store (C is not read
