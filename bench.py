@@ -31,6 +31,8 @@ Other workloads (one JSON line each, same keys):
                           that scales rocprofv3 FETCH_SIZE / WRITE_SIZE
     --workload stencil    five-point stencil sweeps, FP64 haloed grids 2048^2 ..
                           8192^2: grid points per ms and HBM GB/s
+    --workload tune       32 shapes of the serving mix: AUTO dispatch before and
+                          after the autotuner
 """
 from __future__ import annotations
 
@@ -113,7 +115,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--workload", default="dgemm", choices=["dgemm", "skinny", "precision", "serving", "copy", "grouped",
-                                                        "transpose", "sgemm512", "sweep", "stencil"])
+                                                        "transpose", "sgemm512", "sweep", "stencil", "tune"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1"}))
@@ -121,7 +123,8 @@ def main():
     if args.workload != "dgemm":
         {"skinny": bench_skinny, "precision": bench_precision, "serving": bench_serving,
          "copy": bench_copy, "grouped": bench_grouped, "transpose": bench_transpose,
-         "sgemm512": bench_sgemm512, "sweep": bench_sweep, "stencil": bench_stencil}[args.workload](args)
+         "sgemm512": bench_sgemm512, "sweep": bench_sweep, "stencil": bench_stencil,
+         "tune": bench_tune}[args.workload](args)
         return
 
     import torch
@@ -647,6 +650,56 @@ def bench_stencil(args):
                    "algorithmic_bytes_per_launch": 8 * (8192 + 2) ** 2 + 8 * 8192 ** 2,
                    "aggregate_gbs_all_sizes": round(gbs, 1)},
                   cpu, ms_per_step=big["ms"])
+    print(json.dumps(out), flush=True)
+
+
+def bench_tune(args):
+    """The autotuner against the built-in heuristics, on 32 shapes drawn from
+    the serving mix's distribution (M, N, K log-uniform in [16, 4096], 30 %
+    shortA / thinB with small side 8-32, modes uniform, 70 % FP32 / 30 % FP64,
+    seed 4): each shape timed through the AUTO dispatch with an empty table,
+    then blasdsp_autotune for its shape class, then timed again.  The value is
+    the summed time before over the summed time after."""
+    import torch
+    import __graft_entry__
+
+    blas = __graft_entry__.load_package()
+    stream = torch.cuda.current_stream()
+    L = blas.lib()
+    L.blasdsp_tune_clear()
+    reps = max(3, min(args.steps, 10))
+    reqs = serving_requests(blas, n=32, seed=4)
+    cases = []
+    t_heur = t_tuned = flops = 0.0
+    for r in reqs:
+        dt = torch.float64 if r.dtype == blas.FLOAT64 else torch.float32
+        ta, tb = r.transA.decode(), r.transB.decode()
+        A = torch.empty((r.K, r.M) if ta == "T" else (r.M, r.K), dtype=dt, device="cuda")
+        B = torch.empty((r.N, r.K) if tb == "T" else (r.K, r.N), dtype=dt, device="cuda")
+        C = torch.empty((r.M, r.N), dtype=dt, device="cuda")
+        for X, sd in ((A, 1), (B, 2), (C, 3)):
+            blas.fill(X, X.shape[0], X.shape[1], blas.UNIFORM, sd)
+        fn = lambda: blas.gemm(ta, tb, A, B, C, 1.0, 1.0)
+        before = _time_events(fn, reps, stream)
+        choice = blas.autotune(r.dtype, ta, tb, r.M, r.N, r.K)
+        after = _time_events(fn, reps, stream)
+        f = 2.0 * r.M * r.N * r.K
+        cases.append({"dtype": "f64" if r.dtype == blas.FLOAT64 else "f32", "mode": ta + tb, "M": r.M, "N": r.N,
+                      "K": r.K, "us_heuristic": round(before * 1e3, 1), "us_tuned": round(after * 1e3, 1),
+                      "choice": {"algo": "skinny" if choice.algo == blas.ALGO_SKINNY else "tile",
+                                 "tile": choice.tile, "splitk": choice.splitk}})
+        t_heur += before
+        t_tuned += after
+        flops += f
+        del A, B, C
+    L.blasdsp_tune_clear()
+    out = _common(args, round(t_heur / t_tuned, 3), "x (heuristic time / tuned time)", "f32+f64",
+                  {"workload": "32 GEMM shapes drawn from the serving mix (seed 4), AUTO dispatch before and "
+                               "after blasdsp_autotune of each shape class",
+                   "ms_heuristic_total": round(t_heur, 3), "ms_tuned_total": round(t_tuned, 3),
+                   "tflops_heuristic": round(flops / t_heur / 1e9, 2), "tflops_tuned": round(flops / t_tuned / 1e9, 2),
+                   "cases": cases},
+                  None, ms_per_step=round(t_tuned / len(cases), 4))
     print(json.dumps(out), flush=True)
 
 

@@ -110,12 +110,14 @@ bool parse_trans(char t, bool &out)
 
 namespace {
 
+// The heuristic's skinny test: small side <= 64, a long side at least twice
+// the small one, and enough panels and K slices to fill the GPU.
 template <typename T>
 bool skinny_shape(const Gemm<T> &g)
 {
     const int64_t small = g.M <= g.N ? g.M : g.N;
     const int64_t large = g.M <= g.N ? g.N : g.M;
-    return small <= 64 && large >= 2 * small && skinny_fits(g);
+    return small <= 64 && large >= 2 * small && skinny_fits(g) && skinny_fills(g, num_cus());
 }
 
 template <typename T>

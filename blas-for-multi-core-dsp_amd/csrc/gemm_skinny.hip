@@ -708,6 +708,28 @@ bool skinny_fits(const Gemm<T> &g)
     return lmax + 16 < 2147483647.0 && smax + 16 < 2147483647.0;
 }
 
+// Whether the skinny kernel can occupy every resident workgroup slot: its
+// panels times the most K slices it may take (skinny_pick_splitk) reach one
+// full round.  Below that the tile kernel with 64 tiles and split-K runs the
+// shape faster: measured on MI355X over 32 shapes of a mixed GEMM workload,
+// long sides of a few hundred to a few thousand ran 1.1-4.5x slower on the
+// skinny kernel, and the autotuner never picked it there.
+template <typename T>
+bool skinny_fills(const Gemm<T> &g, int num_cus)
+{
+    const int64_t ms = g.M <= g.N ? g.M : g.N;
+    const int64_t nl = g.M <= g.N ? g.N : g.M;
+    const int wg_cu = ms <= 16 ? SkinnyOcc<T, 1, false>::WPE / SkinnyOcc<T, 1, false>::KS
+                               : (ms <= 32 ? SkinnyOcc<T, 2, false>::WPE / SkinnyOcc<T, 2, false>::KS
+                                           : SkinnyOcc<T, 4, false>::WPE / SkinnyOcc<T, 4, false>::KS);
+    const int64_t wgs = (nl + kWaves * kPanel - 1) / (kWaves * kPanel);
+    const int64_t slots = (int64_t)wg_cu * (num_cus > 0 ? num_cus : 256);
+    int64_t max_s = g.K / 512;
+    if (max_s > 64) max_s = 64;
+    if (max_s < 1) max_s = 1;
+    return wgs * max_s >= slots;
+}
+
 template <typename T>
 int skinny_pick_splitk(const Gemm<T> &g, int num_cus)
 {
@@ -763,6 +785,8 @@ hipError_t launch_skinny(const Gemm<T> &g, int splitk, T *work, hipStream_t s)
 
 template bool skinny_fits<double>(const Gemm<double> &);
 template bool skinny_fits<float>(const Gemm<float> &);
+template bool skinny_fills<double>(const Gemm<double> &, int);
+template bool skinny_fills<float>(const Gemm<float> &, int);
 template int skinny_pick_splitk<double>(const Gemm<double> &, int);
 template int skinny_pick_splitk<float>(const Gemm<float> &, int);
 template hipError_t launch_skinny<double>(const Gemm<double> &, int, double *, hipStream_t);

@@ -50,6 +50,10 @@ bool skinny_fits(const Gemm<T> &g);
 // Number of K slices the skinny kernel would use for this shape on this GPU.
 template <typename T>
 int skinny_pick_splitk(const Gemm<T> &g, int num_cus);
+// Whether the skinny kernel's panels and K slices fill one round of the
+// GPU's workgroup slots (else the tile kernel is the faster choice).
+template <typename T>
+bool skinny_fills(const Gemm<T> &g, int num_cus);
 
 // FP32 GEMM on 16-bit matrix cores (gemm_split.hip).  scheme: 0 = BF16 pieces
 // hi/mid/lo with 6 products, 1 = BF16 hi/lo with 3 products, 2 = scaled FP16

@@ -9,9 +9,10 @@
 //
 // Shape class: precision, transpose modes and round(log2) of M, N and K.
 // Candidates: the tile kernel with 64 and 128 tiles, each whole or split into
-// 2, 4 or 8 K slices, and, when the smaller of M and N is at most 64, the
-// skinny kernel with split-K factors 1 .. 64 (slices of at least 256).  Each candidate runs once to warm up and then three times
-// between events on the caller's stream.  The table is a text file:
+// 2, 4 or 8 K slices or into the tile cost model's slice count, and, when the
+// smaller of M and N is at most 64, the skinny kernel with split-K factors
+// 1 .. 64 (slices of at least 256).  Each candidate runs once to warm up and
+// then three times between events on the caller's stream.  The table is a text file:
 //   blasdsp-tune-v1 <dtype> <ta> <tb> <log2 M> <log2 N> <log2 K> <algo> <tile> <splitk> <microseconds>
 #include <math.h>
 #include <stdio.h>
@@ -57,19 +58,25 @@ int autotune(bool ta, bool tb, int64_t M, int64_t N, int64_t K, hipStream_t s, T
     if (M <= 0 || N <= 0 || K <= 0) return BLASDSP_ERR_INVALID;
     std::vector<TuneChoice> cands;
     int max_split = 1;
+    Gemm<T> shape{};
+    shape.ta = ta; shape.tb = tb;
+    shape.M = M; shape.N = N; shape.K = K;
+    shape.lda = ta ? M : K; shape.ldb = tb ? K : N; shape.ldc = N;
     // tile kernel: 64 and 128 tiles, each whole or split into 2 .. 8 K slices
-    // of at least 256
+    // of at least 256, plus the slice count of the tile kernel's own cost
+    // model when it is none of those
     for (int bm : {64, 128}) {
         for (int sk = 1; sk <= 8; sk *= 2) {
             if (sk > 1 && K / sk < 256) break;
             cands.push_back(TuneChoice{BLASDSP_ALGO_TILE, bm, sk, 0});
             max_split = sk > max_split ? sk : max_split;
         }
+        const int model = tile_pick_splitk(shape, bm, num_cus());
+        if (model > 1 && (model & (model - 1)) != 0) {
+            cands.push_back(TuneChoice{BLASDSP_ALGO_TILE, bm, model, 0});
+            max_split = model > max_split ? model : max_split;
+        }
     }
-    Gemm<T> shape{};
-    shape.ta = ta; shape.tb = tb;
-    shape.M = M; shape.N = N; shape.K = K;
-    shape.lda = ta ? M : K; shape.ldb = tb ? K : N; shape.ldc = N;
     if (skinny_fits(shape)) {
         for (int sk = 1; sk <= 64; sk *= 2) {
             if (sk > 1 && K / sk < 256) break;

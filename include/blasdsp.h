@@ -53,8 +53,11 @@ BLASDSP_API int blasdsp_last_hip_error(void);
 /* ---- general entry points ----------------------------------------------- */
 
 /* Dispatches by shape: the MFMA tile kernel for general shapes, the
- * bandwidth-bound skinny kernel when M or N is small.  Replaces the per-mode
- * __global__ functions of dgemm_release_V1.1/include/dgemm_kernel.h:11-38. */
+ * bandwidth-bound skinny kernel when M or N is at most 64 and the other side
+ * is long enough for the skinny kernel's panels and K slices to fill the GPU;
+ * a shape class recorded by blasdsp_autotune takes its recorded choice.
+ * Replaces the per-mode __global__ functions of
+ * dgemm_release_V1.1/include/dgemm_kernel.h:11-38. */
 BLASDSP_API int blasdsp_dgemm(char transA, char transB, int64_t M, int64_t N, int64_t K,
                   double alpha, const double *A, int64_t lda,
                   const double *B, int64_t ldb, double beta,
