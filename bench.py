@@ -33,6 +33,8 @@ Other workloads (one JSON line each, same keys):
                           8192^2: grid points per ms and HBM GB/s
     --workload tune       32 shapes of the serving mix: AUTO dispatch before and
                           after the autotuner
+    --workload batch      512 requests of the serving mix in one grouped
+                          persistent launch vs one call per request
 """
 from __future__ import annotations
 
@@ -115,7 +117,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--workload", default="dgemm", choices=["dgemm", "skinny", "precision", "serving", "copy", "grouped",
-                                                        "transpose", "sgemm512", "sweep", "stencil", "tune"])
+                                                        "transpose", "sgemm512", "sweep", "stencil", "tune",
+                                                        "batch"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1"}))
@@ -124,7 +127,7 @@ def main():
         {"skinny": bench_skinny, "precision": bench_precision, "serving": bench_serving,
          "copy": bench_copy, "grouped": bench_grouped, "transpose": bench_transpose,
          "sgemm512": bench_sgemm512, "sweep": bench_sweep, "stencil": bench_stencil,
-         "tune": bench_tune}[args.workload](args)
+         "tune": bench_tune, "batch": bench_batch}[args.workload](args)
         return
 
     import torch
@@ -400,6 +403,55 @@ def bench_grouped(args):
                   {"workload": "500 FP32 GEMMs, M/N/K uniform in [16, 256], NN",
                    "ms_grouped": round(ms_group, 4), "ms_one_launch_each": round(ms_each, 4),
                    "speedup": round(ms_each / ms_group, 2)},
+                  None, ms_per_step=round(ms_group, 4))
+    print(json.dumps(out), flush=True)
+
+
+def bench_batch(args):
+    """One batch of the serving mix through the grouped persistent launch: 512
+    requests drawn from the serving distribution (M, N, K log-uniform in
+    [16, 4096], 30 % shortA / thinB with small side 8-32, modes uniform, 70 %
+    FP32 / 30 % FP64, seed 4), operands resident in HBM, C += op(A) op(B).
+    Compared with one AUTO call per request issued back to back from Python
+    (wall time on the GPU's clock, launch gaps included) and with the sum of
+    each call's own GPU time when issued alone (no gaps, no overlap)."""
+    import torch
+    import __graft_entry__
+
+    blas = __graft_entry__.load_package()
+    stream = torch.cuda.current_stream()
+    reqs = serving_requests(blas, n=512, seed=4)
+    probs = []
+    for i, r in enumerate(reqs):
+        dt = torch.float64 if r.dtype == blas.FLOAT64 else torch.float32
+        ta, tb = r.transA.decode(), r.transB.decode()
+        A = torch.empty((r.K, r.M) if ta == "T" else (r.M, r.K), dtype=dt, device="cuda")
+        B = torch.empty((r.N, r.K) if tb == "T" else (r.K, r.N), dtype=dt, device="cuda")
+        C = torch.empty((r.M, r.N), dtype=dt, device="cuda")
+        for X, sd in ((A, 3 * i), (B, 3 * i + 1), (C, 3 * i + 2)):
+            blas.fill(X, X.shape[0], X.shape[1], blas.UNIFORM, sd)
+        probs.append((ta, tb, A, B, C))
+    descs = [blas.desc(ta, tb, A, B, C, 1.0, 1.0) for ta, tb, A, B, C in probs]
+    flops = sum(2.0 * r.M * r.N * r.K for r in reqs)
+    reps = max(3, min(args.steps, 10))
+    plan = blas.GroupPlan(len(descs))
+    ms_group = _time_events(lambda: plan.run(descs), reps, stream)
+    ms_wall = _time_events(lambda: [blas.gemm(ta, tb, A, B, C, 1.0, 1.0) for ta, tb, A, B, C in probs], reps,
+                           stream)
+    ms_alone = 0.0
+    for ta, tb, A, B, C in probs:
+        ms_alone += _time_events(lambda: blas.gemm(ta, tb, A, B, C, 1.0, 1.0), 3, stream)
+    plan.close()
+    n = len(reqs)
+    out = _common(args, round(n / ms_group * 1e3, 1), "GEMMs/s", "f32+f64",
+                  {"workload": "512 requests of the serving mix (seed 4) in one grouped persistent launch "
+                               "(hipGraph replay), operands resident, C += op(A) op(B)",
+                   "ms_grouped": round(ms_group, 3), "ms_one_call_each_wall": round(ms_wall, 3),
+                   "ms_one_call_each_gpu_sum": round(ms_alone, 3),
+                   "speedup_vs_wall": round(ms_wall / ms_group, 2),
+                   "speedup_vs_gpu_sum": round(ms_alone / ms_group, 2),
+                   "tflops_grouped": round(flops / ms_group / 1e9, 2),
+                   "gemms_per_s_one_call_each_wall": round(n / ms_wall * 1e3, 1)},
                   None, ms_per_step=round(ms_group, 4))
     print(json.dumps(out), flush=True)
 

@@ -34,6 +34,18 @@ namespace {
 
 using tile::Cfg;
 
+// Waves per workgroup: 128 tiles take the 8-wave body (32 x 64 per wave) of
+// the per-call kernel, 64 tiles 4 waves (32 x 32 each).  With the group's
+// problem lookup the 8-wave FP64 body spills 4-25 registers; on 512 requests
+// of a mixed workload (M, N, K log-uniform in [16, 4096], FP32 and FP64) it
+// still ran 3 % faster than 4 waves for FP64, and 8 waves for both
+// precisions 7 % faster than 4 for both.
+template <typename T, int BM>
+constexpr int group_nw()
+{
+    return BM == 128 ? 8 : 4;
+}
+
 template <typename T, int BM, bool TA, bool TB>
 __device__ __forceinline__ void run_tile(const GroupProblem &P, int64_t local, T *lds)
 {
@@ -45,15 +57,16 @@ __device__ __forceinline__ void run_tile(const GroupProblem &P, int64_t local, T
     g.B = (const T *)P.B; g.ldb = P.ldb;
     g.C = (T *)P.C; g.ldc = P.ldc;
     const int64_t tm = local / P.tiles_n, tn = local % P.tiles_n;
-    tile::tile_body<T, BM, TA, TB>(g, tm * BM, tn * BM, P.a_vec, P.b_vec, lds);
+    tile::tile_body<T, BM, TA, TB, group_nw<T, BM>()>(g, tm * BM, tn * BM, P.a_vec, P.b_vec, lds);
 }
 
 // MODE < 0: the problem's transpose mode is read per tile; else fixed.
 template <typename T, int BM, int MODE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Cfg<T, BM>::WPE, Cfg<T, BM>::WPE)))
+__global__ void __launch_bounds__((64 * group_nw<T, BM>()))
+__attribute__((amdgpu_waves_per_eu(Cfg<T, BM, group_nw<T, BM>()>::WPE, Cfg<T, BM, group_nw<T, BM>()>::WPE)))
 grouped_kernel(const GroupHeader *__restrict__ hdr, const GroupProblem *__restrict__ probs)
 {
-    __shared__ __attribute__((aligned(16))) T lds[Cfg<T, BM>::LDS_ELEMS];
+    __shared__ __attribute__((aligned(16))) T lds[Cfg<T, BM, group_nw<T, BM>()>::LDS_ELEMS];
     const int w = group_list(sizeof(T) == 8, BM, MODE < 0 ? 0 : MODE);
     const int count = hdr->count[w];
     const int64_t total = hdr->total_tiles[w];
@@ -82,14 +95,24 @@ grouped_kernel(const GroupHeader *__restrict__ hdr, const GroupProblem *__restri
 int grouped_grid(int w, int num_cus)
 {
     const int cus = num_cus > 0 ? num_cus : 256;
-    int wpe;
+    int wpe, nw;
     switch (w) {
-    case 0: wpe = Cfg<float, 128>::WPE; break;
-    case 1: wpe = Cfg<float, 64>::WPE; break;
-    case 2: wpe = Cfg<double, 64>::WPE; break;
-    default: wpe = Cfg<double, 128>::WPE; break;
+    case 0: wpe = Cfg<float, 128, group_nw<float, 128>()>::WPE; nw = group_nw<float, 128>(); break;
+    case 1: wpe = Cfg<float, 64, group_nw<float, 64>()>::WPE; nw = group_nw<float, 64>(); break;
+    case 2: wpe = Cfg<double, 64, group_nw<double, 64>()>::WPE; nw = group_nw<double, 64>(); break;
+    default: wpe = Cfg<double, 128, group_nw<double, 128>()>::WPE; nw = group_nw<double, 128>(); break;
     }
-    return cus * wpe;  // 4 waves per workgroup, 4 SIMDs per CU
+    return cus * wpe * 4 / nw;  // resident workgroups: waves per SIMD x 4 SIMDs / waves per workgroup
+}
+
+int grouped_block(int w)
+{
+    switch (w) {
+    case 0: return 64 * group_nw<float, 128>();
+    case 1: return 64 * group_nw<float, 64>();
+    case 2: return 64 * group_nw<double, 64>();
+    default: return 64 * group_nw<double, 128>();
+    }
 }
 
 void *grouped_kernel_ptr(int w)

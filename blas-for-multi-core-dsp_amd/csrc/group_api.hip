@@ -64,7 +64,7 @@ hipKernelNodeParams node_params(blasdsp_group_plan *p, int w, int grid)
     hipKernelNodeParams kp{};
     kp.func = grouped_kernel_ptr(w);
     kp.gridDim = dim3(grid);
-    kp.blockDim = dim3(256);
+    kp.blockDim = dim3(grouped_block(w));
     kp.sharedMemBytes = 0;
     kp.kernelParams = a.ptrs;
     kp.extra = nullptr;
@@ -124,6 +124,21 @@ int create_plan(int capacity, blasdsp_group_plan **out)
     return BLASDSP_SUCCESS;
 }
 
+// Tile edge of one problem of a group.  A group has parallelism across its
+// problems, so the choice is the cheaper cover of the problem's C: tiles x
+// relative tile cost, a 128 tile costing kTile128Cost 64 tiles (its four
+// times the work runs on a larger, more efficient per-wave tile).  On 512
+// requests of a mixed workload (M, N, K log-uniform in [16, 4096]) costs 2.8
+// to 3.2 ran 1.5 % faster than 128 tiles only when M and N exceed 96; 3.6
+// was 2.5 % slower, 4.0 35 % slower.
+constexpr double kTile128Cost = 3.0;
+int group_tile(int64_t M, int64_t N)
+{
+    const double c64 = (double)((M + 63) / 64) * (double)((N + 63) / 64);
+    const double c128 = (double)((M + 127) / 128) * (double)((N + 127) / 128) * kTile128Cost;
+    return c128 <= c64 ? 128 : 64;
+}
+
 template <typename T>
 int to_problem(const blasdsp_gemm_desc &d, GroupProblem &p, bool &skip)
 {
@@ -145,7 +160,7 @@ int to_problem(const blasdsp_gemm_desc &d, GroupProblem &p, bool &skip)
     if (tune_lookup(sizeof(T) == 8, g.ta, g.tb, d.M, d.N, d.K, &c) && c.algo == BLASDSP_ALGO_TILE)
         p.bm = c.tile;
     else
-        p.bm = (d.M <= 96 || d.N <= 96) ? 64 : 128;
+        p.bm = group_tile(d.M, d.N);
     return BLASDSP_SUCCESS;
 }
 

@@ -43,6 +43,8 @@ struct GroupHeader {
 
 // Persistent grid of list w's kernel (its occupancy times the CU count).
 int grouped_grid(int w, int num_cus);
+// Threads per workgroup of list w's kernel.
+int grouped_block(int w);
 // Workgroups of one list's kernel that can hold a tile at once, for sizing
 // the grid of a small list.
 void *grouped_kernel_ptr(int w);
