@@ -179,17 +179,23 @@ __device__ __forceinline__ bool full_tile(const Gemm<T> &g, int64_t m0, int64_t 
     return a_vec && b_vec && m0 + Cf::BM <= g.M && n0 + Cf::BM <= g.N && (ke - kb) % Cf::BK == 0;
 }
 
-// Whether a wave raises its issue priority (s_setprio 1) over the MFMA
-// section of each K tile, back to 0 before its LDS stores and the barrier.
-// Measured on MI355X with the 8-wave 128 tile, per transpose mode, on two
-// boxes with bit-identical results: FP64 4096^3 NT +3.7 %, TN +5.0 %, NN
-// -2.1 %, TT -0.5 %; FP32 8192^3 TN +2.8 %, TT +1.2 %, NN -2.0 %, NT -3.0 %.
-// Raising it for the whole K tile changed nothing.  Only the modes that gain
-// use it.
+// Wave issue priority (s_setprio 1) over one section of each K tile, per
+// precision and transpose mode, where it measured faster on MI355X with the
+// 8-wave 128 tile (interleaved A/B on four boxes, bit-identical results):
+//   over the MFMA section (mfma_prio): FP64 TN +4.5-5 %, FP32 TN +2.6-2.8 %,
+//     TT +1.2 %; for FP64 NN/NT/TT and FP32 NN/NT it cost 1-3 %;
+//   over the LDS stores and the barrier (store_prio): FP64 NN +1.0 %, NT
+//     +1.8-5.5 %, TT +1.2 %; for FP64 TN and FP32 NT/TN it cost 3-5 %.
+// Raising it for the whole K tile, or over the global loads, changed nothing.
 template <typename T, int BM, int NW, bool TA, bool TB>
 constexpr bool mfma_prio()
 {
-    return BM == 128 && NW == 8 && (sizeof(T) == 8 ? TA != TB : TA);
+    return BM == 128 && NW == 8 && (sizeof(T) == 8 ? (TA && !TB) : TA);
+}
+template <typename T, int BM, int NW, bool TA, bool TB>
+constexpr bool store_prio()
+{
+    return BM == 128 && NW == 8 && sizeof(T) == 8 && !(TA && !TB);
 }
 
 // One BM x BM tile of C at (m0, n0) with NW waves, over K range [kb, ke)
@@ -269,11 +275,13 @@ __device__ __forceinline__ void tile_body(const Gemm<T> &g, int64_t m0, int64_t 
         }
 
         if constexpr (mfma_prio<T, BM, NW, TA, TB>()) __builtin_amdgcn_s_setprio(0);
+        if constexpr (store_prio<T, BM, NW, TA, TB>()) __builtin_amdgcn_s_setprio(1);
         if (more) {
             sa.store(stage(cur ^ 1, 0), tid);
             sb.store(stage(cur ^ 1, 1), tid);
         }
         __syncthreads();
+        if constexpr (store_prio<T, BM, NW, TA, TB>()) __builtin_amdgcn_s_setprio(0);
     }
 
     const T alpha = g.alpha, beta = g.beta;
