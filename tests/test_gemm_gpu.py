@@ -66,8 +66,13 @@ def test_tile_kernel_modes(orc, blas, dev, prec, mode, shape):
 @pytest.mark.parametrize("mode", ["NN", "NT", "TN", "TT"])
 @pytest.mark.parametrize("shape", [(8, 300, 1000), (16, 64, 16), (32, 1000, 777), (40, 129, 64),
                                    (1, 70, 33), (300, 8, 1000), (1000, 32, 777), (129, 40, 64), (70, 1, 33),
-                                   (32, 100, 20), (300, 20, 5)])
+                                   (32, 100, 20), (300, 20, 5), (20, 1030, 1000), (1030, 20, 1000)])
 def test_skinny_kernel_modes(orc, blas, dev, prec, mode, shape):
+    """Every mode against the oracle.  The 20 x 1030 x 1000 shapes (and the
+    32-row ones) have whole workgroups of full panels that share the S block
+    through LDS with k-major L, next to a partial one that does not; with
+    K = 1000 the FP32 K-split waves get 16 and 15 full blocks, so one of them
+    finishes past the shared blocks."""
     M, N, K = shape
     got, ref = run_case(orc, blas, dev, prec, mode[0], mode[1], M, N, K, 1.5, -0.5, blas.ALGO_SKINNY)
     err, _ = orc.normwise(got, ref)
