@@ -40,6 +40,8 @@
 // is split into slices (grid.y, skinny_pick_splitk); each slice writes its
 // partial O to a workspace and a second kernel sums the slices in fixed order
 // (deterministic, no atomics) and applies C = alpha * sum + beta * C.
+#include <type_traits>
+
 #include "internal.h"
 
 namespace blasdsp {
@@ -271,7 +273,8 @@ __device__ __forceinline__ T bload(const T *base, uint32_t off)
 // checks and no lane branches: every block issues the same loads, so the
 // wait before a block's MFMAs counts exactly the loads issued after it.
 // n0 and k0 are wave-uniform.
-template <typename T, int MT, int SM, bool L_KC>
+// WITH_S false: L only (S comes from the workgroup's shared chunk).
+template <typename T, int MT, int SM, bool L_KC, bool WITH_S = true>
 __device__ __forceinline__ void load_full(Block<T, MT, SM != 0, L_KC> &b, const SkinnyArgs<T> &a,
                                           const LaneOff<MT> &o, int64_t n0, int64_t k0)
 {
@@ -292,6 +295,7 @@ __device__ __forceinline__ void load_full(Block<T, MT, SM != 0, L_KC> &b, const 
             for (int h = 0; h < 4 / V; ++h)
                 b.l[j * (4 / V) + h] = bload_vec<T, kAuxNT>(a.L + (k0 + j) * a.ldl + n0 + h * 16 * V, o.l);
     }
+    if constexpr (!WITH_S) return;
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
         if (SM != 0) {
@@ -482,6 +486,77 @@ constexpr int skinny_lds_elems()
     return st > red ? st : (red > 0 ? red : 1);
 }
 
+// f(std::integral_constant<int, 0>{}) .. f(std::integral_constant<int, N - 1>{})
+template <int N, int I = 0, typename F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<N, I + 1>(f);
+    }
+}
+
+// Shared S in chunks (32 rows, k-major L, S K-contiguous and 16-byte
+// aligned).  Every wave of a workgroup (of one K part) multiplies the same S
+// block, which at 32 rows is half the size of the wave's L block; read by
+// each wave, it puts half as many bytes again through the memory pipeline as
+// L.  Measured on MI355X at K = 8192, long side 65536: pointing every lane's S
+// load at one address (a timing-only copy) ran the k-major 32-row cases 6-14 %
+// faster.  Shared through LDS block by block, behind a barrier per block, they
+// ran 1-9 % slower: with one workgroup per CU every wave then waits for the
+// slowest one's loads at every block.  So here the workgroup shares S in
+// chunks of SHB blocks with one barrier per chunk:
+//   * the chunk (32 rows x CHK k) sits in LDS as [row][k] (rows padded by 16
+//     bytes, so the 16-byte operand reads of 8 consecutive rows hit distinct
+//     banks), two slots per K part;
+//   * at block b of chunk cc (position p = b % SHB) each wave writes its
+//     staged vector p of chunk cc + 1 to slot (cc + 1) & 1, loads vector p of
+//     chunk cc + 2 into that staging register, and reads its S operands of
+//     block b from slot cc & 1;
+//   * at p = 0 a barrier: every wave has finished chunk cc - 1, so slot
+//     (cc + 1) & 1 = (cc - 1) & 1 is free and slot cc & 1 is complete.
+// One 16-byte load per lane and block, the same at every block, so the wait
+// before each block's MFMAs still counts a fixed number of loads.  Measured
+// (interleaved A/B, bit-identical): shortA NN / thinB TT at 32 rows +10-11 %
+// FP32 and +8.5-9 % FP64.
+template <typename T, int MT, int SM, bool L_KC>
+__host__ __device__ constexpr bool skinny_share()
+{
+    return MT == 2 && !L_KC && SM == 2;
+}
+
+template <typename T>
+struct ShChunk {
+    using B_ = Blk<T, 1>;
+    static constexpr int V = B_::V, KB = B_::KB, RUN = B_::RUN, RV = B_::RV;
+    static constexpr int D = SkinnyOcc<T, 2, false>::D;
+    static constexpr int SHB = 2 * (D + 1);              // blocks per chunk: whole ring rounds
+    static constexpr int CHK = SHB * KB;                 // k per chunk
+    static constexpr int ROW = CHK + 16 / (int)sizeof(T);
+    static constexpr int SLOT = 32 * ROW;                // elements per slot
+    static constexpr int VPR = CHK / V;                  // 16-byte vectors per chunk row
+    static_assert(32 * VPR == SHB * kWaves * 64, "one vector per lane and block");
+};
+
+// Vector v of the chunk at kc: row v / VPR (rows past Ms read row Ms - 1),
+// k = kc + (v % VPR) V; past kend (the end of the shared blocks, whose data
+// is never used) it reads k = kc instead, which is inside the range.
+template <typename T>
+__device__ __forceinline__ typename SK<T>::vec_t sh_load(const SkinnyArgs<T> &a, int64_t kc, int64_t kend, int v)
+{
+    using C_ = ShChunk<T>;
+    const int row = v / C_::VPR, col = v % C_::VPR;
+    const int64_t r = row < a.Ms ? row : a.Ms - 1;
+    const int64_t kk = kc + (int64_t)col * C_::V + C_::V <= kend ? (int64_t)col * C_::V : 0;
+    return bload_vec<T, 0>(a.S + kc, (uint32_t)((r * a.lds + kk) * (int64_t)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ void sh_store(T *slot, int v, typename SK<T>::vec_t x)
+{
+    using C_ = ShChunk<T>;
+    *reinterpret_cast<typename SK<T>::vec_t *>(slot + (v / C_::VPR) * C_::ROW + (v % C_::VPR) * C_::V) = x;
+}
+
 template <typename T, int MT, int SM, bool L_KC, bool TRANS_OUT>
 __global__ void __launch_bounds__(256 * skinny_ks<T>(MT, L_KC))
 __attribute__((amdgpu_waves_per_eu(SkinnyOcc<T, MT, L_KC>::WPE, SkinnyOcc<T, MT, L_KC>::WPE)))
@@ -504,13 +579,23 @@ skinny_kernel(SkinnyArgs<T> a)
     int64_t ke = min(a.K, kb + a.kslice);
 
     constexpr int KB = Blk<T, lines_of<T>(L_KC, MT)>::KB;
+    constexpr bool SHARE = skinny_share<T, MT, SM, L_KC>();
+    // full K blocks that every wave of the workgroup walks in step (shared S)
+    int64_t nbs = 0;
     if (KS == 2) {
         // the first wave of a panel takes the first floor(blocks / 2) K
         // blocks, the second the rest (with the ragged block, if any)
         const int64_t kmid = kb + ((ke - kb + KB - 1) / KB / 2) * KB;
+        if (SHARE) nbs = min((kmid - kb) / KB, (ke - kmid) / KB);
         if (half == 0) ke = kmid;
         else kb = kmid;
+    } else if (SHARE) {
+        nbs = (ke - kb) / KB;
     }
+    // Shared S needs every panel of the workgroup full, so that every wave
+    // walks nbs blocks and meets the same barriers; workgroup-uniform.
+    const bool shared = SHARE && nbs > 0 && a.s_vec && a.l_vec &&
+                        ((int64_t)blockIdx.x + 1) * kWaves * kPanel <= a.Nl;
 
     acc_t acc[MT][4];
 #pragma unroll
@@ -524,6 +609,8 @@ skinny_kernel(SkinnyArgs<T> a)
     // K-contiguous L is staged through the wave's own LDS rows.
     __shared__ __attribute__((aligned(16))) T stage[skinny_lds_elems<T, MT, L_KC>()];
     T *const w = stage + (L_KC ? wave * Stage<T, MT>::ELEMS : 0);
+    __shared__ __attribute__((aligned(16))) T sshare[SHARE ? KS * 2 * ShChunk<T>::SLOT : 1];
+    int64_t k0 = kfull;
     if (active) {
         // Full blocks: unmasked loads into a ring of D + 1 register blocks.
         // Block j lives in b[j % (D + 1)]; while block j is multiplied, the loads
@@ -531,20 +618,108 @@ skinny_kernel(SkinnyArgs<T> a)
         // whole rounds of D + 1 blocks, so every ring index is a compile-time
         // constant and no block is copied between registers.
         constexpr int D = SkinnyOcc<T, MT, L_KC>::D;
-        const int64_t nb = (kfull - kb) / KB;
+        if constexpr (SHARE) {
+            if (shared) {
+                using C_ = ShChunk<T>;
+                using vec_t = typename SK<T>::vec_t;
+                constexpr int SHB = C_::SHB;
+                static_assert(C_::D == D, "a chunk is two ring rounds");
+                const LaneOff<MT> o = lane_offsets<T, MT, SM, L_KC>(a, c, g);
+                T *const sh = sshare + half * (2 * C_::SLOT);
+                const int64_t kend = kb + nbs * KB, nch = (nbs + SHB - 1) / SHB;
+                // chunk start; chunks past the last reload chunk 0 (never used)
+                auto chunk_k = [&](int64_t cc) { return cc < nch ? kb + cc * C_::CHK : kb; };
+                const int vb = pw * 64 + lane;  // this lane's vector p is p * 256 + vb
+                vec_t stg[SHB];
+                // chunk 0 to slot 0, chunk 1 to the staging registers; block
+                // 0's barrier orders the slot writes before any read
+#pragma unroll
+                for (int p = 0; p < SHB; ++p) stg[p] = sh_load<T>(a, chunk_k(0), kend, p * 256 + vb);
+#pragma unroll
+                for (int p = 0; p < SHB; ++p) sh_store<T>(sh, p * 256 + vb, stg[p]);
+#pragma unroll
+                for (int p = 0; p < SHB; ++p) stg[p] = sh_load<T>(a, chunk_k(1), kend, p * 256 + vb);
+                // block j at chunk position p = j % SHB (a compile-time value)
+                auto use = [&](Block<T, MT, S_KC, L_KC> &x, int64_t j, auto pc) {
+                    constexpr int p = decltype(pc)::value;
+                    const int64_t cc = j / SHB;
+                    if constexpr (p == 0) __syncthreads();
+                    sh_store<T>(sh + ((cc + 1) & 1) * C_::SLOT, p * 256 + vb, stg[p]);
+                    stg[p] = sh_load<T>(a, chunk_k(cc + 2), kend, p * 256 + vb);
+                    const T *src = sh + (cc & 1) * C_::SLOT + c * C_::ROW + p * KB + g * C_::RUN;
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                        for (int q = 0; q < C_::RV; ++q)
+                            x.s_run[mt][q] = *reinterpret_cast<const vec_t *>(src + mt * 16 * C_::ROW + q * C_::V);
+                    mma_block<T, MT, S_KC, L_KC>(acc, x);
+                    __builtin_amdgcn_sched_barrier(0);
+                };
+                Block<T, MT, S_KC, L_KC> b[D + 1];
+                int64_t i = 0;
+                if (nbs >= SHB + D) {
+                    // unconditional first loads: see the ring below
+#pragma unroll
+                    for (int u = 0; u < D; ++u) {
+                        load_full<T, MT, SM, L_KC, false>(b[u], a, o, n0, kb + u * KB);
+                        __builtin_amdgcn_sched_barrier(0);
+                    }
+                    for (; i + SHB + D <= nbs; i += SHB) {
+                        static_for<SHB>([&](auto uc) {
+                            constexpr int u = decltype(uc)::value;
+                            load_full<T, MT, SM, L_KC, false>(b[(u + D) % (D + 1)], a, o, n0,
+                                                              kb + (i + u + D) * KB);
+                            use(b[u % (D + 1)], i + u, uc);
+                        });
+                    }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < D; ++u)
+                        if (u < nbs) load_full<T, MT, SM, L_KC, false>(b[u], a, o, n0, kb + u * KB);
+                }
+                // the last nbs - i < SHB + D blocks; blocks i .. i + D - 1 are loaded
+                static_for<SHB + D>([&](auto uc) {
+                    constexpr int u = decltype(uc)::value;
+                    if (i + u < nbs) {
+                        if (i + u + D < nbs)
+                            load_full<T, MT, SM, L_KC, false>(b[(u + D) % (D + 1)], a, o, n0, kb + (i + u + D) * KB);
+                        use(b[u % (D + 1)], i + u, std::integral_constant<int, u % SHB>{});
+                    }
+                });
+                k0 = kend;
+            }
+        }
+        const int64_t nb = shared ? 0 : (kfull - kb) / KB;
         if (nb > 0) {
             const LaneOff<MT> o = lane_offsets<T, MT, SM, L_KC>(a, c, g);
             Block<T, MT, S_KC, L_KC> b[D + 1];
-#pragma unroll
-            for (int u = 0; u < D; ++u)
-                if (u < nb) load_full<T, MT, SM, L_KC>(b[u], a, o, n0, kb + u * KB);
             int64_t i = 0;
-            for (; i + 2 * D < nb; i += D + 1) {
+            if (nb > 2 * D) {
+                // The first D blocks are loaded unconditionally and in ring
+                // order (a scheduling barrier after each) on the way into the
+                // steady loop.  The wait before each block's MFMAs is counted
+                // at compile time, and at the loop head the compiler takes the
+                // fewest loads issued after a block on any incoming path;
+                // with the first blocks' loads reordered or conditional, the
+                // first block of every round waited for all but the newest
+                // block in flight (the other resident waves covered it: the
+                // fix measured within noise at K = 8192).
 #pragma unroll
-                for (int u = 0; u <= D; ++u) {
-                    load_full<T, MT, SM, L_KC>(b[(u + D) % (D + 1)], a, o, n0, kb + (i + u + D) * KB);
-                    consume_block<T, MT, S_KC, L_KC>(acc, b[u], w, c, g);
+                for (int u = 0; u < D; ++u) {
+                    load_full<T, MT, SM, L_KC>(b[u], a, o, n0, kb + u * KB);
+                    __builtin_amdgcn_sched_barrier(0);
                 }
+                for (; i + 2 * D < nb; i += D + 1) {
+#pragma unroll
+                    for (int u = 0; u <= D; ++u) {
+                        load_full<T, MT, SM, L_KC>(b[(u + D) % (D + 1)], a, o, n0, kb + (i + u + D) * KB);
+                        consume_block<T, MT, S_KC, L_KC>(acc, b[u], w, c, g);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < D; ++u)
+                    if (u < nb) load_full<T, MT, SM, L_KC>(b[u], a, o, n0, kb + u * KB);
             }
             // the last nb - i <= 2D blocks; blocks i .. i + D - 1 are loaded
 #pragma unroll
@@ -556,8 +731,8 @@ skinny_kernel(SkinnyArgs<T> a)
                 }
             }
         }
-        int64_t k0 = kfull;
-        // Ragged tail (partial panel or partial K block): masked loads.
+        // Ragged tail (partial panel or partial K block, and in shared mode
+        // the blocks past the shared ones): masked loads.
         for (; k0 < ke; k0 += KB) {
             Block<T, MT, S_KC, L_KC> b;
             load_tail<T, MT, SM, L_KC>(b, a, n0, k0, ke, c, g);
@@ -758,7 +933,8 @@ bool skinny_fits(const Gemm<T> &g)
     const double CPR = lines * 128 / s / V, RPI = 64 / CPR;
     const double kg = 3 * (l_kc ? V : RUN);
     const double lmax = l_kc ? ((RPI - 1) * (double)a.ldl + (CPR - 1) * V) * s : (3 * RUN * (double)a.ldl + 15 * V) * s;
-    const double smax = s_kc ? (63.0 * (double)a.lds + kg) * s : (kg * (double)a.lds + 63) * s;
+    // (K-contiguous S: the shared chunk loads reach up to 256 elements along k)
+    const double smax = s_kc ? (63.0 * (double)a.lds + (kg > 256 ? kg : 256)) * s : (kg * (double)a.lds + 63) * s;
     // a 16-byte access at the offset stays inside the buffer resource's 2^31 - 1 bytes
     return lmax + 16 < 2147483647.0 && smax + 16 < 2147483647.0;
 }

@@ -34,12 +34,16 @@ def stored(orc, rows, cols, trans, dist, seed, prec, pad=3):
     return orc.matrix(rows, cols, dist, seed, dtype=dtype_np(prec), ld=scols + pad, trans=trans)
 
 
-def run_case(orc, blas, dev, prec, ta, tb, M, N, K, alpha, beta, algo, dist=0, seeds=(1, 2, 3), splitk=0):
+def run_case(orc, blas, dev, prec, ta, tb, M, N, K, alpha, beta, algo, dist=0, seeds=(1, 2, 3), splitk=0,
+             pad=3, cpad=5):
+    """Stored A and B with leading dimension (stored columns + pad), C with
+    N + cpad: the default pads leave rows unaligned (bounds-checked paths);
+    pad = cpad = 4 aligns every row to 16 bytes (the unmasked paths)."""
     import torch
 
-    A = stored(orc, M, K, ta == "T", dist, seeds[0], prec)
-    B = stored(orc, K, N, tb == "T", dist, seeds[1], prec)
-    C0 = orc.matrix(M, N, dist if dist in (3, 4) else 0, seeds[2], dtype=dtype_np(prec), ld=N + 5)
+    A = stored(orc, M, K, ta == "T", dist, seeds[0], prec, pad=pad)
+    B = stored(orc, K, N, tb == "T", dist, seeds[1], prec, pad=pad)
+    C0 = orc.matrix(M, N, dist if dist in (3, 4) else 0, seeds[2], dtype=dtype_np(prec), ld=N + cpad)
     ref = orc.gemm_xp(ta, tb, A, B, C0, alpha, beta)
     dA, dB, dC = to_dev(A, dev), to_dev(B, dev), to_dev(C0, dev)
     blas.gemm(ta, tb, dA, dB, dC, alpha, beta, algo=algo, splitk=splitk)
@@ -67,14 +71,15 @@ def test_tile_kernel_modes(orc, blas, dev, prec, mode, shape):
 @pytest.mark.parametrize("shape", [(8, 300, 1000), (16, 64, 16), (32, 1000, 777), (40, 129, 64),
                                    (1, 70, 33), (300, 8, 1000), (1000, 32, 777), (129, 40, 64), (70, 1, 33),
                                    (32, 100, 20), (300, 20, 5), (20, 1030, 1000), (1030, 20, 1000)])
-def test_skinny_kernel_modes(orc, blas, dev, prec, mode, shape):
-    """Every mode against the oracle.  The 20 x 1030 x 1000 shapes (and the
-    32-row ones) have whole workgroups of full panels that share the S block
-    through LDS with k-major L, next to a partial one that does not; with
-    K = 1000 the FP32 K-split waves get 16 and 15 full blocks, so one of them
-    finishes past the shared blocks."""
+@pytest.mark.parametrize("pad", [3, 4])
+def test_skinny_kernel_modes(orc, blas, dev, prec, mode, shape, pad):
+    """Every mode against the oracle, with rows unaligned (pad 3: bounds-checked
+    loads) and 16-byte aligned (pad 4: unmasked buffer loads for whole panels
+    and K blocks, 16-byte stores of C).  With K = 1000 the FP32 32-row K-split
+    waves of the 20 x 1030 shapes walk 16 and 15 blocks."""
     M, N, K = shape
-    got, ref = run_case(orc, blas, dev, prec, mode[0], mode[1], M, N, K, 1.5, -0.5, blas.ALGO_SKINNY)
+    got, ref = run_case(orc, blas, dev, prec, mode[0], mode[1], M, N, K, 1.5, -0.5, blas.ALGO_SKINNY,
+                        pad=pad, cpad=pad + 2 if pad == 3 else pad)
     err, _ = orc.normwise(got, ref)
     assert err <= (TOL64 if prec == "d" else tol32(K)), err
 
@@ -123,6 +128,26 @@ def test_skinny_splitk(orc, blas, dev, splitk):
                             splitk=splitk)
         err, _ = orc.normwise(got, ref)
         assert err <= (TOL64 if prec == "d" else tol32(1111)), (prec, err)
+
+
+@pytest.mark.parametrize("prec", ["d", "s"])
+@pytest.mark.parametrize("splitk", [1, 3])
+@pytest.mark.parametrize("case", [("NN", 32, 1100, 4000), ("TT", 1100, 32, 4000), ("NN", 28, 2100, 1000),
+                                  ("TT", 2100, 28, 1000)])
+def test_skinny_shared_small_operand(orc, blas, dev, prec, splitk, case):
+    """32-row skinny calls with the small operand K-contiguous and aligned
+    (shortA NN, thinB TT) and k-major streamed operand: whole workgroups of
+    full panels share the small operand through LDS in chunks of blocks.  K =
+    4000 and 1000 give several chunks with a partial last one, a ragged last
+    K block, and with 3 slices a different chunk walk per slice; the partial
+    last workgroup (long side 1100 / 2100) takes the per-wave path.  Rows
+    16-byte aligned (unmasked loads, vector stores of C).  Against the
+    oracle."""
+    mode, M, N, K = case
+    got, ref = run_case(orc, blas, dev, prec, mode[0], mode[1], M, N, K, 1.5, -0.5, blas.ALGO_SKINNY,
+                        splitk=splitk, pad=4, cpad=4)
+    err, _ = orc.normwise(got, ref)
+    assert err <= (TOL64 if prec == "d" else tol32(K)), err
 
 
 @pytest.mark.parametrize("algo", ["tile", "skinny"])
