@@ -496,8 +496,8 @@ __device__ __forceinline__ void static_for(F &&f)
     }
 }
 
-// Shared S in chunks (32 rows, k-major L, S K-contiguous and 16-byte
-// aligned).  Every wave of a workgroup (of one K part) multiplies the same S
+// Shared S in chunks (32 rows, k-major L, S 16-byte aligned with 16-byte
+// aligned rows; stored m-contiguous, also Ms a multiple of the vector).  Every wave of a workgroup (of one K part) multiplies the same S
 // block, which at 32 rows is half the size of the wave's L block; read by
 // each wave, it puts half as many bytes again through the memory pipeline as
 // L.  Measured on MI355X at K = 8192, long side 65536: pointing every lane's S
@@ -508,7 +508,8 @@ __device__ __forceinline__ void static_for(F &&f)
 // chunks of SHB blocks with one barrier per chunk:
 //   * the chunk (32 rows x CHK k) sits in LDS as [row][k] (rows padded by 16
 //     bytes, so the 16-byte operand reads of 8 consecutive rows hit distinct
-//     banks), two slots per K part;
+//     banks), two slots per K part; S stored m-contiguous is read as V rows
+//     at one k per 16 bytes and written to LDS element by element;
 //   * at block b of chunk cc (position p = b % SHB) each wave writes its
 //     staged vector p of chunk cc + 1 to slot (cc + 1) & 1, loads vector p of
 //     chunk cc + 2 into that staging register, and reads its S operands of
@@ -517,12 +518,13 @@ __device__ __forceinline__ void static_for(F &&f)
 //     (cc + 1) & 1 = (cc - 1) & 1 is free and slot cc & 1 is complete.
 // One 16-byte load per lane and block, the same at every block, so the wait
 // before each block's MFMAs still counts a fixed number of loads.  Measured
-// (interleaved A/B, bit-identical): shortA NN / thinB TT at 32 rows +10-11 %
-// FP32 and +8.5-9 % FP64.
+// (interleaved A/B, bit-identical), 32 rows: S K-contiguous (shortA NN,
+// thinB TT) +10-11 % FP32 and +8.5-9 % FP64; S m-contiguous (shortA TN,
+// thinB TN) +4.7 % FP32 and +1-6 % FP64.
 template <typename T, int MT, int SM, bool L_KC>
 __host__ __device__ constexpr bool skinny_share()
 {
-    return MT == 2 && !L_KC && SM == 2;
+    return MT == 2 && !L_KC && SM != 1;
 }
 
 template <typename T>
@@ -538,23 +540,40 @@ struct ShChunk {
     static_assert(32 * VPR == SHB * kWaves * 64, "one vector per lane and block");
 };
 
-// Vector v of the chunk at kc: row v / VPR (rows past Ms read row Ms - 1),
-// k = kc + (v % VPR) V; past kend (the end of the shared blocks, whose data
-// is never used) it reads k = kc instead, which is inside the range.
-template <typename T>
+// Vector v of the chunk at kc.  S K-contiguous (SM 2): row v / VPR (rows
+// past Ms read row Ms - 1), k = kc + (v % VPR) V.  S m-contiguous (SM 0):
+// k = kc + v / (32 / V), rows (v % (32 / V)) V .. + V - 1 (a vector wholly
+// past Ms reads rows from 0; Ms is a multiple of V).  Past kend (the end of
+// the shared blocks, whose data is never used) it reads k = kc instead, which
+// is inside the range.
+template <typename T, int SM>
 __device__ __forceinline__ typename SK<T>::vec_t sh_load(const SkinnyArgs<T> &a, int64_t kc, int64_t kend, int v)
 {
     using C_ = ShChunk<T>;
-    const int row = v / C_::VPR, col = v % C_::VPR;
-    const int64_t r = row < a.Ms ? row : a.Ms - 1;
-    const int64_t kk = kc + (int64_t)col * C_::V + C_::V <= kend ? (int64_t)col * C_::V : 0;
-    return bload_vec<T, 0>(a.S + kc, (uint32_t)((r * a.lds + kk) * (int64_t)sizeof(T)));
+    constexpr int64_t s = sizeof(T);
+    if constexpr (SM == 2) {
+        const int row = v / C_::VPR, col = v % C_::VPR;
+        const int64_t r = row < a.Ms ? row : a.Ms - 1;
+        const int64_t kk = kc + (int64_t)col * C_::V + C_::V <= kend ? (int64_t)col * C_::V : 0;
+        return bload_vec<T, 0>(a.S + kc, (uint32_t)((r * a.lds + kk) * s));
+    } else {
+        const int kk = v / (32 / C_::V), r0 = (v % (32 / C_::V)) * C_::V;
+        const int64_t r = r0 < a.Ms ? r0 : 0;
+        const int64_t kq = kc + kk < kend ? kk : 0;
+        return bload_vec<T, 0>(a.S + kc * a.lds, (uint32_t)((kq * a.lds + r) * s));
+    }
 }
-template <typename T>
+template <typename T, int SM>
 __device__ __forceinline__ void sh_store(T *slot, int v, typename SK<T>::vec_t x)
 {
     using C_ = ShChunk<T>;
-    *reinterpret_cast<typename SK<T>::vec_t *>(slot + (v / C_::VPR) * C_::ROW + (v % C_::VPR) * C_::V) = x;
+    if constexpr (SM == 2) {
+        *reinterpret_cast<typename SK<T>::vec_t *>(slot + (v / C_::VPR) * C_::ROW + (v % C_::VPR) * C_::V) = x;
+    } else {
+        const int kk = v / (32 / C_::V), r0 = (v % (32 / C_::V)) * C_::V;
+#pragma unroll
+        for (int e = 0; e < C_::V; ++e) slot[(r0 + e) * C_::ROW + kk] = x[e];
+    }
 }
 
 template <typename T, int MT, int SM, bool L_KC, bool TRANS_OUT>
@@ -594,7 +613,7 @@ skinny_kernel(SkinnyArgs<T> a)
     }
     // Shared S needs every panel of the workgroup full, so that every wave
     // walks nbs blocks and meets the same barriers; workgroup-uniform.
-    const bool shared = SHARE && nbs > 0 && a.s_vec && a.l_vec &&
+    const bool shared = SHARE && nbs > 0 && a.s_vec && a.l_vec && (SM != 0 || a.Ms % SK<T>::V == 0) &&
                         ((int64_t)blockIdx.x + 1) * kWaves * kPanel <= a.Nl;
 
     acc_t acc[MT][4];
@@ -634,24 +653,31 @@ skinny_kernel(SkinnyArgs<T> a)
                 // chunk 0 to slot 0, chunk 1 to the staging registers; block
                 // 0's barrier orders the slot writes before any read
 #pragma unroll
-                for (int p = 0; p < SHB; ++p) stg[p] = sh_load<T>(a, chunk_k(0), kend, p * 256 + vb);
+                for (int p = 0; p < SHB; ++p) stg[p] = sh_load<T, SM>(a, chunk_k(0), kend, p * 256 + vb);
 #pragma unroll
-                for (int p = 0; p < SHB; ++p) sh_store<T>(sh, p * 256 + vb, stg[p]);
+                for (int p = 0; p < SHB; ++p) sh_store<T, SM>(sh, p * 256 + vb, stg[p]);
 #pragma unroll
-                for (int p = 0; p < SHB; ++p) stg[p] = sh_load<T>(a, chunk_k(1), kend, p * 256 + vb);
+                for (int p = 0; p < SHB; ++p) stg[p] = sh_load<T, SM>(a, chunk_k(1), kend, p * 256 + vb);
                 // block j at chunk position p = j % SHB (a compile-time value)
                 auto use = [&](Block<T, MT, S_KC, L_KC> &x, int64_t j, auto pc) {
                     constexpr int p = decltype(pc)::value;
                     const int64_t cc = j / SHB;
                     if constexpr (p == 0) __syncthreads();
-                    sh_store<T>(sh + ((cc + 1) & 1) * C_::SLOT, p * 256 + vb, stg[p]);
-                    stg[p] = sh_load<T>(a, chunk_k(cc + 2), kend, p * 256 + vb);
+                    sh_store<T, SM>(sh + ((cc + 1) & 1) * C_::SLOT, p * 256 + vb, stg[p]);
+                    stg[p] = sh_load<T, SM>(a, chunk_k(cc + 2), kend, p * 256 + vb);
                     const T *src = sh + (cc & 1) * C_::SLOT + c * C_::ROW + p * KB + g * C_::RUN;
 #pragma unroll
                     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-                        for (int q = 0; q < C_::RV; ++q)
-                            x.s_run[mt][q] = *reinterpret_cast<const vec_t *>(src + mt * 16 * C_::ROW + q * C_::V);
+                        for (int q = 0; q < C_::RV; ++q) {
+                            const vec_t sv = *reinterpret_cast<const vec_t *>(src + mt * 16 * C_::ROW + q * C_::V);
+                            if constexpr (S_KC) {
+                                x.s_run[mt][q] = sv;
+                            } else {
+#pragma unroll
+                                for (int e = 0; e < C_::V; ++e) x.s_scl[mt][q * C_::V + e] = sv[e];
+                            }
+                        }
                     mma_block<T, MT, S_KC, L_KC>(acc, x);
                     __builtin_amdgcn_sched_barrier(0);
                 };
@@ -933,8 +959,9 @@ bool skinny_fits(const Gemm<T> &g)
     const double CPR = lines * 128 / s / V, RPI = 64 / CPR;
     const double kg = 3 * (l_kc ? V : RUN);
     const double lmax = l_kc ? ((RPI - 1) * (double)a.ldl + (CPR - 1) * V) * s : (3 * RUN * (double)a.ldl + 15 * V) * s;
-    // (K-contiguous S: the shared chunk loads reach up to 256 elements along k)
-    const double smax = s_kc ? (63.0 * (double)a.lds + (kg > 256 ? kg : 256)) * s : (kg * (double)a.lds + 63) * s;
+    // (the shared chunk loads reach up to 256 elements along k)
+    const double smax = s_kc ? (63.0 * (double)a.lds + (kg > 256 ? kg : 256)) * s
+                             : ((kg > 256 ? kg : 256) * (double)a.lds + 63) * s;
     // a 16-byte access at the offset stays inside the buffer resource's 2^31 - 1 bytes
     return lmax + 16 < 2147483647.0 && smax + 16 < 2147483647.0;
 }
