@@ -496,60 +496,69 @@ __device__ __forceinline__ void static_for(F &&f)
     }
 }
 
-// Shared S in chunks (32 rows, k-major L, S 16-byte aligned with 16-byte
-// aligned rows; stored m-contiguous, also Ms a multiple of the vector).  Every wave of a workgroup (of one K part) multiplies the same S
-// block, which at 32 rows is half the size of the wave's L block; read by
-// each wave, it puts half as many bytes again through the memory pipeline as
-// L.  Measured on MI355X at K = 8192, long side 65536: pointing every lane's S
-// load at one address (a timing-only copy) ran the k-major 32-row cases 6-14 %
-// faster.  Shared through LDS block by block, behind a barrier per block, they
-// ran 1-9 % slower: with one workgroup per CU every wave then waits for the
-// slowest one's loads at every block.  So here the workgroup shares S in
-// chunks of SHB blocks with one barrier per chunk:
-//   * the chunk (32 rows x CHK k) sits in LDS as [row][k] (rows padded by 16
-//     bytes, so the 16-byte operand reads of 8 consecutive rows hit distinct
-//     banks), two slots per K part; S stored m-contiguous is read as V rows
-//     at one k per 16 bytes and written to LDS element by element;
-//   * at block b of chunk cc (position p = b % SHB) each wave writes its
-//     staged vector p of chunk cc + 1 to slot (cc + 1) & 1, loads vector p of
-//     chunk cc + 2 into that staging register, and reads its S operands of
-//     block b from slot cc & 1;
+// Shared S in chunks (16 or 32 rows: MT = 1 with Ms > 8, or MT = 2; k-major
+// L; S 16-byte aligned with 16-byte aligned rows and, stored m-contiguous, Ms
+// a multiple of the vector).  Every wave of a workgroup (of one K part)
+// multiplies the same S block, which at 32 rows is half the size of the
+// wave's L block (a quarter at 16); read by each wave, it puts that many
+// bytes again through the memory pipeline.  Measured on MI355X at K = 8192,
+// long side 65536: pointing every lane's S load at one address (a
+// timing-only copy) ran the k-major cases 6-14 % faster at 32 rows, 2-11 % at
+// 16 and not measurably at 8.  Shared through LDS block by block, behind a
+// barrier per block, the 32-row cases ran 1-9 % slower: with one workgroup
+// per CU every wave then waits for the slowest one's loads at every block.
+// So here the workgroup shares S in chunks of SHB blocks (two ring rounds)
+// with one barrier per chunk:
+//   * the chunk (ROWS = 16 MT rows x CHK k) sits in LDS as [row][k] (rows
+//     padded by 16 bytes, so the 16-byte operand reads of 8 consecutive rows
+//     hit distinct banks), two slots per K part; S stored m-contiguous is
+//     read as V rows at one k per 16 bytes and written to LDS element by
+//     element;
+//   * every lane stages STG 16-byte vectors of each chunk, one at every
+//     EVERY-th block (every block at 32 rows, every other at 16): at block b
+//     of chunk cc (position p = b % SHB, p a multiple of EVERY) a wave writes
+//     its staged vector p / EVERY of chunk cc + 1 to slot (cc + 1) & 1 and
+//     loads the same vector of chunk cc + 2 into that register; at every
+//     block it reads its S operands from slot cc & 1;
 //   * at p = 0 a barrier: every wave has finished chunk cc - 1, so slot
 //     (cc + 1) & 1 = (cc - 1) & 1 is free and slot cc & 1 is complete.
-// One 16-byte load per lane and block, the same at every block, so the wait
-// before each block's MFMAs still counts a fixed number of loads.  Measured
-// (interleaved A/B, bit-identical), 32 rows: S K-contiguous (shortA NN,
-// thinB TT) +10-11 % FP32 and +8.5-9 % FP64; S m-contiguous (shortA TN,
-// thinB TN) +4.7 % FP32 and +1-6 % FP64.
+// The loads at each chunk position are fixed, so the wait before each
+// block's MFMAs still counts a known number of loads.  Measured (interleaved
+// A/B, bit-identical), 32 rows: S K-contiguous (shortA NN, thinB TT) +10-11 %
+// FP32 and +8.5-9 % FP64; S m-contiguous (shortA TN, thinB TN) +4.7 % FP32 and
+// +1-6 % FP64; 16 rows, the same eight cases: +2-8 %.
 template <typename T, int MT, int SM, bool L_KC>
 __host__ __device__ constexpr bool skinny_share()
 {
-    return MT == 2 && !L_KC && SM != 1;
+    return MT <= 2 && !L_KC && SM != 1;
 }
 
-template <typename T>
+template <typename T, int MT>
 struct ShChunk {
     using B_ = Blk<T, 1>;
     static constexpr int V = B_::V, KB = B_::KB, RUN = B_::RUN, RV = B_::RV;
-    static constexpr int D = SkinnyOcc<T, 2, false>::D;
+    static constexpr int ROWS = 16 * MT;
+    static constexpr int D = SkinnyOcc<T, MT, false>::D;
     static constexpr int SHB = 2 * (D + 1);              // blocks per chunk: whole ring rounds
     static constexpr int CHK = SHB * KB;                 // k per chunk
     static constexpr int ROW = CHK + 16 / (int)sizeof(T);
-    static constexpr int SLOT = 32 * ROW;                // elements per slot
+    static constexpr int SLOT = ROWS * ROW;              // elements per slot
     static constexpr int VPR = CHK / V;                  // 16-byte vectors per chunk row
-    static_assert(32 * VPR == SHB * kWaves * 64, "one vector per lane and block");
+    static constexpr int STG = ROWS * VPR / (kWaves * 64);  // vectors per lane per chunk
+    static constexpr int EVERY = SHB / STG;              // blocks per staged vector
+    static_assert(STG * kWaves * 64 == ROWS * VPR && STG * EVERY == SHB, "whole vectors per lane");
 };
 
 // Vector v of the chunk at kc.  S K-contiguous (SM 2): row v / VPR (rows
 // past Ms read row Ms - 1), k = kc + (v % VPR) V.  S m-contiguous (SM 0):
-// k = kc + v / (32 / V), rows (v % (32 / V)) V .. + V - 1 (a vector wholly
-// past Ms reads rows from 0; Ms is a multiple of V).  Past kend (the end of
-// the shared blocks, whose data is never used) it reads k = kc instead, which
-// is inside the range.
-template <typename T, int SM>
+// k = kc + v / (ROWS / V), rows (v % (ROWS / V)) V .. + V - 1 (a vector
+// wholly past Ms reads rows from 0; Ms is a multiple of V).  Past kend (the
+// end of the shared blocks, whose data is never used) it reads k = kc
+// instead, which is inside the range.
+template <typename T, int MT, int SM>
 __device__ __forceinline__ typename SK<T>::vec_t sh_load(const SkinnyArgs<T> &a, int64_t kc, int64_t kend, int v)
 {
-    using C_ = ShChunk<T>;
+    using C_ = ShChunk<T, MT>;
     constexpr int64_t s = sizeof(T);
     if constexpr (SM == 2) {
         const int row = v / C_::VPR, col = v % C_::VPR;
@@ -557,20 +566,20 @@ __device__ __forceinline__ typename SK<T>::vec_t sh_load(const SkinnyArgs<T> &a,
         const int64_t kk = kc + (int64_t)col * C_::V + C_::V <= kend ? (int64_t)col * C_::V : 0;
         return bload_vec<T, 0>(a.S + kc, (uint32_t)((r * a.lds + kk) * s));
     } else {
-        const int kk = v / (32 / C_::V), r0 = (v % (32 / C_::V)) * C_::V;
+        const int kk = v / (C_::ROWS / C_::V), r0 = (v % (C_::ROWS / C_::V)) * C_::V;
         const int64_t r = r0 < a.Ms ? r0 : 0;
         const int64_t kq = kc + kk < kend ? kk : 0;
         return bload_vec<T, 0>(a.S + kc * a.lds, (uint32_t)((kq * a.lds + r) * s));
     }
 }
-template <typename T, int SM>
+template <typename T, int MT, int SM>
 __device__ __forceinline__ void sh_store(T *slot, int v, typename SK<T>::vec_t x)
 {
-    using C_ = ShChunk<T>;
+    using C_ = ShChunk<T, MT>;
     if constexpr (SM == 2) {
         *reinterpret_cast<typename SK<T>::vec_t *>(slot + (v / C_::VPR) * C_::ROW + (v % C_::VPR) * C_::V) = x;
     } else {
-        const int kk = v / (32 / C_::V), r0 = (v % (32 / C_::V)) * C_::V;
+        const int kk = v / (C_::ROWS / C_::V), r0 = (v % (C_::ROWS / C_::V)) * C_::V;
 #pragma unroll
         for (int e = 0; e < C_::V; ++e) slot[(r0 + e) * C_::ROW + kk] = x[e];
     }
@@ -614,7 +623,7 @@ skinny_kernel(SkinnyArgs<T> a)
     // Shared S needs every panel of the workgroup full, so that every wave
     // walks nbs blocks and meets the same barriers; workgroup-uniform.
     const bool shared = SHARE && nbs > 0 && a.s_vec && a.l_vec && (SM != 0 || a.Ms % SK<T>::V == 0) &&
-                        ((int64_t)blockIdx.x + 1) * kWaves * kPanel <= a.Nl;
+                        (MT == 2 || a.Ms > 8) && ((int64_t)blockIdx.x + 1) * kWaves * kPanel <= a.Nl;
 
     acc_t acc[MT][4];
 #pragma unroll
@@ -628,7 +637,8 @@ skinny_kernel(SkinnyArgs<T> a)
     // K-contiguous L is staged through the wave's own LDS rows.
     __shared__ __attribute__((aligned(16))) T stage[skinny_lds_elems<T, MT, L_KC>()];
     T *const w = stage + (L_KC ? wave * Stage<T, MT>::ELEMS : 0);
-    __shared__ __attribute__((aligned(16))) T sshare[SHARE ? KS * 2 * ShChunk<T>::SLOT : 1];
+    constexpr int SH_MT = SHARE ? MT : 1;  // chunk geometry exists for MT 1 and 2 only
+    __shared__ __attribute__((aligned(16))) T sshare[SHARE ? KS * 2 * ShChunk<T, SH_MT>::SLOT : 1];
     int64_t k0 = kfull;
     if (active) {
         // Full blocks: unmasked loads into a ring of D + 1 register blocks.
@@ -639,32 +649,35 @@ skinny_kernel(SkinnyArgs<T> a)
         constexpr int D = SkinnyOcc<T, MT, L_KC>::D;
         if constexpr (SHARE) {
             if (shared) {
-                using C_ = ShChunk<T>;
+                using C_ = ShChunk<T, SH_MT>;
                 using vec_t = typename SK<T>::vec_t;
-                constexpr int SHB = C_::SHB;
+                constexpr int SHB = C_::SHB, STG = C_::STG, EVERY = C_::EVERY;
                 static_assert(C_::D == D, "a chunk is two ring rounds");
                 const LaneOff<MT> o = lane_offsets<T, MT, SM, L_KC>(a, c, g);
                 T *const sh = sshare + half * (2 * C_::SLOT);
                 const int64_t kend = kb + nbs * KB, nch = (nbs + SHB - 1) / SHB;
                 // chunk start; chunks past the last reload chunk 0 (never used)
                 auto chunk_k = [&](int64_t cc) { return cc < nch ? kb + cc * C_::CHK : kb; };
-                const int vb = pw * 64 + lane;  // this lane's vector p is p * 256 + vb
-                vec_t stg[SHB];
+                const int vb = pw * 64 + lane;  // this lane's vector q is q * 256 + vb
+                vec_t stg[STG];
                 // chunk 0 to slot 0, chunk 1 to the staging registers; block
                 // 0's barrier orders the slot writes before any read
 #pragma unroll
-                for (int p = 0; p < SHB; ++p) stg[p] = sh_load<T, SM>(a, chunk_k(0), kend, p * 256 + vb);
+                for (int q = 0; q < STG; ++q) stg[q] = sh_load<T, MT, SM>(a, chunk_k(0), kend, q * 256 + vb);
 #pragma unroll
-                for (int p = 0; p < SHB; ++p) sh_store<T, SM>(sh, p * 256 + vb, stg[p]);
+                for (int q = 0; q < STG; ++q) sh_store<T, MT, SM>(sh, q * 256 + vb, stg[q]);
 #pragma unroll
-                for (int p = 0; p < SHB; ++p) stg[p] = sh_load<T, SM>(a, chunk_k(1), kend, p * 256 + vb);
+                for (int q = 0; q < STG; ++q) stg[q] = sh_load<T, MT, SM>(a, chunk_k(1), kend, q * 256 + vb);
                 // block j at chunk position p = j % SHB (a compile-time value)
                 auto use = [&](Block<T, MT, S_KC, L_KC> &x, int64_t j, auto pc) {
                     constexpr int p = decltype(pc)::value;
                     const int64_t cc = j / SHB;
                     if constexpr (p == 0) __syncthreads();
-                    sh_store<T, SM>(sh + ((cc + 1) & 1) * C_::SLOT, p * 256 + vb, stg[p]);
-                    stg[p] = sh_load<T, SM>(a, chunk_k(cc + 2), kend, p * 256 + vb);
+                    if constexpr (p % EVERY == 0) {
+                        constexpr int q = p / EVERY;
+                        sh_store<T, MT, SM>(sh + ((cc + 1) & 1) * C_::SLOT, q * 256 + vb, stg[q]);
+                        stg[q] = sh_load<T, MT, SM>(a, chunk_k(cc + 2), kend, q * 256 + vb);
+                    }
                     const T *src = sh + (cc & 1) * C_::SLOT + c * C_::ROW + p * KB + g * C_::RUN;
 #pragma unroll
                     for (int mt = 0; mt < MT; ++mt)
@@ -959,9 +972,9 @@ bool skinny_fits(const Gemm<T> &g)
     const double CPR = lines * 128 / s / V, RPI = 64 / CPR;
     const double kg = 3 * (l_kc ? V : RUN);
     const double lmax = l_kc ? ((RPI - 1) * (double)a.ldl + (CPR - 1) * V) * s : (3 * RUN * (double)a.ldl + 15 * V) * s;
-    // (the shared chunk loads reach up to 256 elements along k)
-    const double smax = s_kc ? (63.0 * (double)a.lds + (kg > 256 ? kg : 256)) * s
-                             : ((kg > 256 ? kg : 256) * (double)a.lds + 63) * s;
+    // (the shared chunk loads reach up to 512 elements along k)
+    const double smax = s_kc ? (63.0 * (double)a.lds + (kg > 512 ? kg : 512)) * s
+                             : ((kg > 512 ? kg : 512) * (double)a.lds + 63) * s;
     // a 16-byte access at the offset stays inside the buffer resource's 2^31 - 1 bytes
     return lmax + 16 < 2147483647.0 && smax + 16 < 2147483647.0;
 }

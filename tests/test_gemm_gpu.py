@@ -134,9 +134,11 @@ def test_skinny_splitk(orc, blas, dev, splitk):
 @pytest.mark.parametrize("splitk", [1, 3])
 @pytest.mark.parametrize("case", [("NN", 32, 1100, 4000), ("TT", 1100, 32, 4000), ("NN", 28, 2100, 1000),
                                   ("TT", 2100, 28, 1000), ("TN", 32, 1100, 4000), ("TN", 1100, 32, 4000),
-                                  ("TN", 28, 2100, 1000), ("TN", 2100, 28, 1000), ("TN", 30, 1100, 1000)])
+                                  ("TN", 28, 2100, 1000), ("TN", 2100, 28, 1000), ("TN", 30, 1100, 1000),
+                                  ("NN", 16, 1100, 4000), ("TT", 1100, 16, 4000), ("TN", 12, 2100, 1000),
+                                  ("TN", 2100, 16, 1000)])
 def test_skinny_shared_small_operand(orc, blas, dev, prec, splitk, case):
-    """32-row skinny calls with aligned rows and a k-major streamed operand:
+    """16- and 32-row skinny calls with aligned rows and a k-major streamed operand:
     whole workgroups of full panels share the small operand through LDS in
     chunks of blocks, read K-contiguous (shortA NN, thinB TT) or m-contiguous
     (shortA TN, thinB TN; 30 rows is not a whole number of FP32 vectors, so
