@@ -9,52 +9,50 @@
 // vector memory, transposes them with 16 x 16 (or 32 x 32) register
 // micro-kernels and DMAs them back.
 //
-// Here one workgroup moves a 64 x 64 tile (128 x 128 for aligned FP32):
-// 16-byte loads along the rows of A, scalar writes into a padded LDS tile at
-// the transposed position, then 16-byte stores along the rows of B.  Every element crosses HBM once each
-// way; the kernel is HBM-bound (algorithmic bytes = 2 * M * N * sizeof(T)).
-// Edge tiles and unaligned operands take element-wise accesses.
+// Here one workgroup moves a 64 x 64 tile (128 x 128 for FP32 with 16-byte
+// aligned rows): 16-byte loads along the rows of A (8-byte for FP32 rows that
+// are only 8-byte aligned), scalar writes into a padded LDS tile at the
+// transposed position, then stores of the same width along the rows of B.
+// Every element crosses HBM once each way; the kernel is HBM-bound
+// (algorithmic bytes = 2 * M * N * sizeof(T)).  Edge tiles and unaligned
+// operands take element-wise accesses.
 #include "api_common.h"
 
 namespace blasdsp {
 namespace {
 
-template <typename T>
-struct TVec;
-template <>
-struct TVec<float> {
-    typedef float v __attribute__((ext_vector_type(4)));
-    static constexpr int V = 4;
-};
-template <>
-struct TVec<double> {
-    typedef double v __attribute__((ext_vector_type(2)));
-    static constexpr int V = 2;
+// VW elements per access in full tiles: 16 bytes when both operands allow it
+// (rows 16-byte aligned), 8 bytes for FP32 with rows 8-byte aligned, else 1
+// (element-wise, with the 16-byte lane mapping).
+template <typename T, int VW>
+struct TAcc {
+    static constexpr int V = VW > 1 ? VW : 16 / (int)sizeof(T);  // elements per lane and row piece
+    typedef T v __attribute__((ext_vector_type(V)));
 };
 
-// Tile edge: 64, or 128 for aligned FP32 (512-byte row pieces each way, as
-// FP64 gets at 64).  Measured on MI355X, FP32: 16384^2 4.15 -> 4.40 TB/s and
-// 4096 x 8192 +2 % with 128; the unaligned 8210 x 8190 (element-wise
-// accesses) ran 10 % slower with 128, so it keeps 64.  FP64 did not change.
-template <typename T, bool VEC>
+// Tile edge: 64, or 128 for FP32 with 16-byte accesses (512-byte row pieces
+// each way, as FP64 gets at 64).  Measured on MI355X, FP32: 16384^2 4.15 ->
+// 4.40 TB/s and 4096 x 8192 +2 % with 128; 8210 x 8190 (8-byte accesses,
+// 3.03 -> 3.28 TB/s against element-wise ones) ran 20 % slower with 128.
+// FP64 did not change.
+template <typename T, int VW>
 constexpr int tile_edge()
 {
-    return sizeof(T) == 4 && VEC ? 128 : 64;
+    return sizeof(T) == 4 && VW == 4 ? 128 : 64;
 }
 
 // A: M x N (lda), B: N x M (ldb).  Tile (by, bx) covers rows by*E.. of A and
-// columns bx*E.. (E = tile_edge).  VEC: both operands 16-byte aligned with ld
-// a multiple of V.
-template <typename T, bool VEC>
+// columns bx*E.. (E = tile_edge).
+template <typename T, int VW>
 __global__ void __launch_bounds__(256) transpose_kernel(const T *__restrict__ A, int64_t lda, T *__restrict__ B,
                                                         int64_t ldb, int64_t M, int64_t N)
 {
-    using vec = typename TVec<T>::v;
-    constexpr int V = TVec<T>::V, kTT = tile_edge<T, VEC>(), VPR = kTT / V;  // vectors per tile row
+    using vec = typename TAcc<T, VW>::v;
+    constexpr int V = TAcc<T, VW>::V, kTT = tile_edge<T, VW>(), VPR = kTT / V;  // vectors per tile row
     __shared__ T tile[kTT][kTT + 1];
     const int tid = threadIdx.x;
     const int64_t r0 = (int64_t)blockIdx.y * kTT, c0 = (int64_t)blockIdx.x * kTT;
-    const bool full = VEC && r0 + kTT <= M && c0 + kTT <= N;
+    const bool full = VW > 1 && r0 + kTT <= M && c0 + kTT <= N;
     // load: thread (row, vector) = (e / VPR, e % VPR), e = tid + 256 i
 #pragma unroll
     for (int i = 0; i < kTT * VPR / 256; ++i) {
@@ -100,14 +98,18 @@ int transpose(int64_t M, int64_t N, const T *A, int64_t lda, T *B, int64_t ldb, 
     const uintptr_t a0 = (uintptr_t)A, a1 = (uintptr_t)(A + (M - 1) * lda + N);
     const uintptr_t b0 = (uintptr_t)B, b1 = (uintptr_t)(B + (N - 1) * ldb + M);
     if (a0 < b1 && b0 < a1) return BLASDSP_ERR_INVALID;
-    constexpr int V = TVec<T>::V;
-    const bool vec = a0 % 16 == 0 && b0 % 16 == 0 && lda % V == 0 && ldb % V == 0;
-    const int64_t kTT = vec ? tile_edge<T, true>() : tile_edge<T, false>();
+    constexpr int V16 = 16 / (int)sizeof(T);
+    const bool vec16 = a0 % 16 == 0 && b0 % 16 == 0 && lda % V16 == 0 && ldb % V16 == 0;
+    const bool vec8 = sizeof(T) == 4 && a0 % 8 == 0 && b0 % 8 == 0 && lda % 2 == 0 && ldb % 2 == 0;
+    const int vw = vec16 ? V16 : (vec8 ? 2 : 1);
+    const int64_t kTT = vw == V16 ? tile_edge<T, V16>() : (vw == 2 ? tile_edge<T, 2>() : tile_edge<T, 1>());
     const int64_t gx = (N + kTT - 1) / kTT, gy = (M + kTT - 1) / kTT;
     if (gx > 0x7fffffffLL || gy > 65535) return BLASDSP_ERR_INVALID;
     const dim3 grid((unsigned)gx, (unsigned)gy);
-    if (vec) hipLaunchKernelGGL((transpose_kernel<T, true>), grid, dim3(256), 0, (hipStream_t)stream, A, lda, B, ldb, M, N);
-    else hipLaunchKernelGGL((transpose_kernel<T, false>), grid, dim3(256), 0, (hipStream_t)stream, A, lda, B, ldb, M, N);
+    const hipStream_t s = (hipStream_t)stream;
+    if (vw == V16) hipLaunchKernelGGL((transpose_kernel<T, V16>), grid, dim3(256), 0, s, A, lda, B, ldb, M, N);
+    else if (vw == 2) hipLaunchKernelGGL((transpose_kernel<T, 2>), grid, dim3(256), 0, s, A, lda, B, ldb, M, N);
+    else hipLaunchKernelGGL((transpose_kernel<T, 1>), grid, dim3(256), 0, s, A, lda, B, ldb, M, N);
     return hip_status(hipGetLastError());
 }
 
