@@ -9,10 +9,11 @@
 // vector memory, transposes them with 16 x 16 (or 32 x 32) register
 // micro-kernels and DMAs them back.
 //
-// Here one workgroup moves a 64 x 64 tile (128 x 128 for FP32 with 16-byte
-// aligned rows): 16-byte loads along the rows of A (8-byte for FP32 rows that
-// are only 8-byte aligned), scalar writes into a padded LDS tile at the
-// transposed position, then stores of the same width along the rows of B.
+// Here one workgroup moves a 64 x 64 tile (128 x 128 on 512 threads for FP32
+// with 16-byte aligned rows): 16-byte loads along the rows of A (8-byte for
+// FP32 rows that are only 8-byte aligned), scalar writes into a padded LDS
+// tile at the transposed position, then stores of the same width along the
+// rows of B.
 // Every element crosses HBM once each way; the kernel is HBM-bound
 // (algorithmic bytes = 2 * M * N * sizeof(T)).  Edge tiles and unaligned
 // operands take element-wise accesses.
@@ -30,33 +31,48 @@ struct TAcc {
     typedef T v __attribute__((ext_vector_type(V)));
 };
 
-// Tile edge: 64, or 128 for FP32 with 16-byte accesses (512-byte row pieces
-// each way, as FP64 gets at 64).  Measured on MI355X, FP32: 16384^2 4.15 ->
-// 4.40 TB/s and 4096 x 8192 +2 % with 128; 8210 x 8190 (8-byte accesses,
-// 3.03 -> 3.28 TB/s against element-wise ones) ran 20 % slower with 128.
-// FP64 did not change.
+// Tile: 64 x 64 on 256 threads, or 128 x 128 on 512 threads for FP32 with
+// 16-byte accesses (512-byte row pieces each way, as FP64 gets at 64, and
+// twice the loads in flight per workgroup).  Measured on MI355X, FP32 16384^2:
+// 4.15 TB/s with 64 x 64, 4.39 with 128 x 128 on 256 threads, 5.16-5.21 on
+// 512 (64 x 128 on 256: 5.09; 128 x 128 on 1024: 4.66); 4096 x 8192 3.9 ->
+// 4.46.  For 8210 x 8190 (8-byte accesses, 3.03 -> 3.28 TB/s against
+// element-wise ones) a 128 tile ran 20 % slower.  FP64 on 512 threads ran
+// 7-9 % slower.
 template <typename T, int VW>
-constexpr int tile_edge()
+constexpr int tile_rows()
 {
     return sizeof(T) == 4 && VW == 4 ? 128 : 64;
 }
-
-// A: M x N (lda), B: N x M (ldb).  Tile (by, bx) covers rows by*E.. of A and
-// columns bx*E.. (E = tile_edge).
 template <typename T, int VW>
-__global__ void __launch_bounds__(256) transpose_kernel(const T *__restrict__ A, int64_t lda, T *__restrict__ B,
-                                                        int64_t ldb, int64_t M, int64_t N)
+constexpr int tile_cols()
+{
+    return sizeof(T) == 4 && VW == 4 ? 128 : 64;
+}
+template <typename T, int VW>
+constexpr int tile_threads()
+{
+    return sizeof(T) == 4 && VW == 4 ? 512 : 256;
+}
+
+// A: M x N (lda), B: N x M (ldb).  Tile (by, bx) covers rows by*TR.. of A and
+// columns bx*TC.. (TR = tile_rows, TC = tile_cols).
+template <typename T, int VW>
+__global__ void __launch_bounds__((tile_threads<T, VW>()))
+transpose_kernel(const T *__restrict__ A, int64_t lda, T *__restrict__ B, int64_t ldb, int64_t M, int64_t N)
 {
     using vec = typename TAcc<T, VW>::v;
-    constexpr int V = TAcc<T, VW>::V, kTT = tile_edge<T, VW>(), VPR = kTT / V;  // vectors per tile row
-    __shared__ T tile[kTT][kTT + 1];
+    constexpr int V = TAcc<T, VW>::V, TR = tile_rows<T, VW>(), TC = tile_cols<T, VW>();
+    constexpr int NT = tile_threads<T, VW>();
+    constexpr int VPR = TC / V, VPC = TR / V;  // vectors per tile row (load), per tile column (store)
+    __shared__ T tile[TC][TR + 1];
     const int tid = threadIdx.x;
-    const int64_t r0 = (int64_t)blockIdx.y * kTT, c0 = (int64_t)blockIdx.x * kTT;
-    const bool full = VW > 1 && r0 + kTT <= M && c0 + kTT <= N;
-    // load: thread (row, vector) = (e / VPR, e % VPR), e = tid + 256 i
+    const int64_t r0 = (int64_t)blockIdx.y * TR, c0 = (int64_t)blockIdx.x * TC;
+    const bool full = VW > 1 && r0 + TR <= M && c0 + TC <= N;
+    // load: thread (row, vector) = (e / VPR, e % VPR), e = tid + NT i
 #pragma unroll
-    for (int i = 0; i < kTT * VPR / 256; ++i) {
-        const int e = tid + 256 * i, r = e / VPR, cv = (e % VPR) * V;
+    for (int i = 0; i < TR * VPR / NT; ++i) {
+        const int e = tid + NT * i, r = e / VPR, cv = (e % VPR) * V;
         vec x;
         if (full) {
             x = __builtin_nontemporal_load(reinterpret_cast<const vec *>(A + (r0 + r) * lda + c0 + cv));
@@ -71,8 +87,8 @@ __global__ void __launch_bounds__(256) transpose_kernel(const T *__restrict__ A,
     __syncthreads();
     // store: row c0 + rr of B holds column c0 + rr of A
 #pragma unroll
-    for (int i = 0; i < kTT * VPR / 256; ++i) {
-        const int e = tid + 256 * i, rr = e / VPR, mv = (e % VPR) * V;
+    for (int i = 0; i < TC * VPC / NT; ++i) {
+        const int e = tid + NT * i, rr = e / VPC, mv = (e % VPC) * V;
         vec x;
 #pragma unroll
         for (int q = 0; q < V; ++q) x[q] = tile[rr][mv + q];
@@ -102,14 +118,17 @@ int transpose(int64_t M, int64_t N, const T *A, int64_t lda, T *B, int64_t ldb, 
     const bool vec16 = a0 % 16 == 0 && b0 % 16 == 0 && lda % V16 == 0 && ldb % V16 == 0;
     const bool vec8 = sizeof(T) == 4 && a0 % 8 == 0 && b0 % 8 == 0 && lda % 2 == 0 && ldb % 2 == 0;
     const int vw = vec16 ? V16 : (vec8 ? 2 : 1);
-    const int64_t kTT = vw == V16 ? tile_edge<T, V16>() : (vw == 2 ? tile_edge<T, 2>() : tile_edge<T, 1>());
-    const int64_t gx = (N + kTT - 1) / kTT, gy = (M + kTT - 1) / kTT;
+    const int64_t tr = vw == V16 ? tile_rows<T, V16>() : 64, tc = vw == V16 ? tile_cols<T, V16>() : 64;
+    const int64_t gx = (N + tc - 1) / tc, gy = (M + tr - 1) / tr;
     if (gx > 0x7fffffffLL || gy > 65535) return BLASDSP_ERR_INVALID;
     const dim3 grid((unsigned)gx, (unsigned)gy);
     const hipStream_t s = (hipStream_t)stream;
-    if (vw == V16) hipLaunchKernelGGL((transpose_kernel<T, V16>), grid, dim3(256), 0, s, A, lda, B, ldb, M, N);
-    else if (vw == 2) hipLaunchKernelGGL((transpose_kernel<T, 2>), grid, dim3(256), 0, s, A, lda, B, ldb, M, N);
-    else hipLaunchKernelGGL((transpose_kernel<T, 1>), grid, dim3(256), 0, s, A, lda, B, ldb, M, N);
+    if (vw == V16)
+        hipLaunchKernelGGL((transpose_kernel<T, V16>), grid, dim3(tile_threads<T, V16>()), 0, s, A, lda, B, ldb, M, N);
+    else if (vw == 2)
+        hipLaunchKernelGGL((transpose_kernel<T, 2>), grid, dim3(tile_threads<T, 2>()), 0, s, A, lda, B, ldb, M, N);
+    else
+        hipLaunchKernelGGL((transpose_kernel<T, 1>), grid, dim3(tile_threads<T, 1>()), 0, s, A, lda, B, ldb, M, N);
     return hip_status(hipGetLastError());
 }
 
