@@ -9,9 +9,9 @@
 // vector memory, transposes them with 16 x 16 (or 32 x 32) register
 // micro-kernels and DMAs them back.
 //
-// Here one workgroup moves a 64 x 64 tile (128 x 128 on 512 threads for FP32
-// with 16-byte aligned rows): 16-byte loads along the rows of A (8-byte for
-// FP32 rows that are only 8-byte aligned), scalar writes into a padded LDS
+// Here one workgroup moves a 128 x 128 tile (64 x 64 for FP32 rows that are
+// only 8-byte aligned and for unaligned operands): 16-byte loads along the
+// rows of A (8-byte for those FP32 rows), scalar writes into a padded LDS
 // tile at the transposed position, then stores of the same width along the
 // rows of B.
 // Every element crosses HBM once each way; the kernel is HBM-bound
@@ -31,28 +31,39 @@ struct TAcc {
     typedef T v __attribute__((ext_vector_type(V)));
 };
 
-// Tile: 64 x 64 on 256 threads, or 128 x 128 on 512 threads for FP32 with
-// 16-byte accesses (512-byte row pieces each way, as FP64 gets at 64, and
-// twice the loads in flight per workgroup).  Measured on MI355X, FP32 16384^2:
-// 4.15 TB/s with 64 x 64, 4.39 with 128 x 128 on 256 threads, 5.16-5.21 on
-// 512 (64 x 128 on 256: 5.09; 128 x 128 on 1024: 4.66); 4096 x 8192 3.9 ->
-// 4.46.  For 8210 x 8190 (8-byte accesses, 3.03 -> 3.28 TB/s against
-// element-wise ones) a 128 tile ran 20 % slower.  FP64 on 512 threads ran
-// 7-9 % slower.
+// Tile shape and workgroup size per access width, measured on MI355X
+// (interleaved A/B over 16384^2, the reference harness's 8210 x 8190 and
+// 4096 x 8192; all exact):
+//   * FP64, 16-byte rows: 128 x 128 on 1024 threads (one workgroup per CU by
+//     LDS).  Against 64 x 64 on 256: 16384^2 5.21 -> 5.55-5.57 TB/s, 8210 x
+//     8190 4.00 -> 4.75-4.77, 4096 x 8192 4.45 -> 5.21-5.27; on 512 threads
+//     it ran 15 % slower, 64 x 128 on 512 gained about half as much;
+//   * FP32, 16-byte rows: 128 x 128 on 512 threads (512-byte row pieces each
+//     way): 16384^2 4.15 (64 x 64) -> 4.39 (on 256 threads) -> 5.15-5.21;
+//     on 1024 threads 4.66;
+//   * FP32, 8-byte rows: 64 x 64 on 512 threads: 8210 x 8190 3.03
+//     (element-wise) -> 3.28 (on 256) -> 3.36-3.39; a 128 tile ran 10-20 %
+//     slower;
+//   * element-wise: 64 x 64 on 256 threads.
+template <typename T, int VW>
+constexpr bool big_tile()
+{
+    return (sizeof(T) == 8 && VW == 2) || (sizeof(T) == 4 && VW == 4);
+}
 template <typename T, int VW>
 constexpr int tile_rows()
 {
-    return sizeof(T) == 4 && VW == 4 ? 128 : 64;
+    return big_tile<T, VW>() ? 128 : 64;
 }
 template <typename T, int VW>
 constexpr int tile_cols()
 {
-    return sizeof(T) == 4 && VW == 4 ? 128 : 64;
+    return big_tile<T, VW>() ? 128 : 64;
 }
 template <typename T, int VW>
 constexpr int tile_threads()
 {
-    return sizeof(T) == 4 && VW == 4 ? 512 : 256;
+    return sizeof(T) == 8 && VW == 2 ? 1024 : (VW > 1 ? 512 : 256);
 }
 
 // A: M x N (lda), B: N x M (ldb).  Tile (by, bx) covers rows by*TR.. of A and
@@ -119,6 +130,7 @@ int transpose(int64_t M, int64_t N, const T *A, int64_t lda, T *B, int64_t ldb, 
     const bool vec8 = sizeof(T) == 4 && a0 % 8 == 0 && b0 % 8 == 0 && lda % 2 == 0 && ldb % 2 == 0;
     const int vw = vec16 ? V16 : (vec8 ? 2 : 1);
     const int64_t tr = vw == V16 ? tile_rows<T, V16>() : 64, tc = vw == V16 ? tile_cols<T, V16>() : 64;
+    static_assert((V16 == 2 || tile_rows<T, 2>() == 64) && tile_rows<T, 1>() == 64, "8-byte and element-wise tiles are 64");
     const int64_t gx = (N + tc - 1) / tc, gy = (M + tr - 1) / tr;
     if (gx > 0x7fffffffLL || gy > 65535) return BLASDSP_ERR_INVALID;
     const dim3 grid((unsigned)gx, (unsigned)gy);
