@@ -13,10 +13,9 @@
 // only 8-byte aligned and for unaligned operands): 16-byte loads along the
 // rows of A (8-byte for those FP32 rows), scalar writes into a padded LDS
 // tile at the transposed position, then stores of the same width along the
-// rows of B.
-// Every element crosses HBM once each way; the kernel is HBM-bound
-// (algorithmic bytes = 2 * M * N * sizeof(T)).  Edge tiles and unaligned
-// operands take element-wise accesses.
+// rows of B.  Every element crosses HBM once each way; the kernel is
+// HBM-bound (algorithmic bytes = 2 * M * N * sizeof(T)).  Edge tiles and
+// unaligned operands take element-wise accesses.
 #include "api_common.h"
 
 namespace blasdsp {
@@ -76,6 +75,7 @@ transpose_kernel(const T *__restrict__ A, int64_t lda, T *__restrict__ B, int64_
     constexpr int V = TAcc<T, VW>::V, TR = tile_rows<T, VW>(), TC = tile_cols<T, VW>();
     constexpr int NT = tile_threads<T, VW>();
     constexpr int VPR = TC / V, VPC = TR / V;  // vectors per tile row (load), per tile column (store)
+    static_assert(TR * VPR % NT == 0 && TC * VPC % NT == 0, "whole vectors per thread");
     __shared__ T tile[TC][TR + 1];
     const int tid = threadIdx.x;
     const int64_t r0 = (int64_t)blockIdx.y * TR, c0 = (int64_t)blockIdx.x * TC;
