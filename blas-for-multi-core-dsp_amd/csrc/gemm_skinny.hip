@@ -103,12 +103,13 @@ constexpr int kPanel = 64;     // O columns per wave
 // the streamed operand is still read exactly once.  Measured on MI355X at
 // K = 8192, long side 65536: 32 rows of S (where the matrix cores must run
 // at ~65 % of peak to keep pace with the stream) gain 1-16 % per case, except
-// FP64 with k-major L, which is 1-3 % slower and keeps one wave; 8 and 16
-// rows gain nothing.
+// FP64 with k-major L, which is 1-3 % slower and keeps one wave.  At 8 and 16
+// rows only FP32 with K-contiguous L gains (0.2-3.5 % per case, interleaved
+// A/B); the other 8/16-row cases gain nothing and keep one wave.
 template <typename T>
 __host__ __device__ constexpr int skinny_ks(int mt, bool l_kc)
 {
-    return mt == 2 && (sizeof(T) == 4 || l_kc) ? 2 : 1;
+    return (mt == 2 && (sizeof(T) == 4 || l_kc)) || (mt == 1 && sizeof(T) == 4 && l_kc) ? 2 : 1;
 }
 
 // 128-byte lines of each K-contiguous row of L per block.  A wave's block
