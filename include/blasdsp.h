@@ -163,7 +163,9 @@ enum blasdsp_algo {
 };
 /* BLASDSP_ALGO_SKINNY takes shapes whose small side is <= 64 and whose
  * leading dimensions keep the kernel's per-lane load offsets below 2^31
- * bytes (about 20 million elements); other shapes run on the tile kernel.
+ * bytes (about 20 million elements for the long operand; about 1 million
+ * FP32 / 0.5 million FP64 for a small operand stored with its short side
+ * contiguous); other shapes run on the tile kernel.
  * splitk: K slices, their partial products added in a fixed order by a
  * second kernel (deterministic).  <= 0: chosen by the library — the tile
  * kernel splits K only when its C tiles fill the GPU's workgroup slots
