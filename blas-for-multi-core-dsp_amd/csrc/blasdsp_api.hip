@@ -110,14 +110,19 @@ bool parse_trans(char t, bool &out)
 
 namespace {
 
-// The heuristic's skinny test: small side <= 64, a long side at least twice
-// the small one, and enough panels and K slices to fill the GPU.
+// The heuristic's skinny test: small side <= 48, a long side at least twice
+// the small one, and enough panels and K slices to fill the GPU.  From 49
+// rows both kernels pad the small side to 64, and there the tile kernel ran
+// as fast or faster on MI355X: over 18 shapes of 64 rows (long side 3024 /
+// 8192 / 20480, K 2048 / 8192 / 20480, FP64 and FP32) at most 2 % slower
+// than the skinny kernel and up to 1.57x faster (FP32 8192 x 64 x 8192).
+// Explicit BLASDSP_ALGO_SKINNY still takes up to 64 rows.
 template <typename T>
 bool skinny_shape(const Gemm<T> &g)
 {
     const int64_t small = g.M <= g.N ? g.M : g.N;
     const int64_t large = g.M <= g.N ? g.N : g.M;
-    return small <= 64 && large >= 2 * small && skinny_fits(g) && skinny_fills(g, num_cus());
+    return small <= 48 && large >= 2 * small && skinny_fits(g) && skinny_fills(g, num_cus());
 }
 
 template <typename T>

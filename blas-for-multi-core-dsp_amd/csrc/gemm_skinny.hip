@@ -1005,11 +1005,15 @@ bool skinny_fills(const Gemm<T> &g, int num_cus)
 template <typename T>
 int skinny_pick_splitk(const Gemm<T> &g, int num_cus)
 {
-    // Cost of a launch in units of one unsplit panel's walk over K: s slices
-    // take ceil(wgs * s / slots) rounds of 1/s each (a round that is not full
-    // leaves CUs idle), plus the partial sums, written once per slice and
-    // read once by the reduction, against the streamed K x Nl.  Fewer slices
-    // win near-ties.
+    // Cost of a launch in units of one full round of unsplit panel walks over
+    // K: s slices take ceil(wgs * s / slots) rounds of 1/s each (a round that
+    // is not full leaves CUs idle), plus the partial sums, written once per
+    // slice and read once by the reduction, against what a full round
+    // streams (slots panels' K x 256 block of L).  Fewer slices win
+    // near-ties.  Pricing the partial sums against the streamed K x Nl
+    // instead overcharged them by slots / wgs: on FP64 3024 x 20480 x 48 it
+    // took 12 slices at 17.9 TFLOP/s where 21 (one full round) run 29.4, and
+    // FP32 3024 x 20480 x 16 took 18 slices at 29.2 where 20 run 31.3.
     const int64_t ms = g.M <= g.N ? g.M : g.N;
     const int64_t nl = g.M <= g.N ? g.N : g.M;
     // resident workgroups per CU: waves per SIMD over the K-split waves per
@@ -1022,10 +1026,12 @@ int skinny_pick_splitk(const Gemm<T> &g, int num_cus)
     int64_t max_s = g.K / 512;  // slices of >= 512
     if (max_s > 64) max_s = 64;
     int best = 1;
-    double best_cost = (double)((wgs + slots - 1) / slots);
+    const double fill = 256.0 / (double)g.K;  // a walk's fixed cost: ring fill and drain, epilogue
+    double best_cost = (double)((wgs + slots - 1) / slots) * (1.0 + fill);
+    const double per_slice = (double)ms * (double)nl / ((double)g.K * (double)slots * (kWaves * kPanel));
     for (int64_t sk = 2; sk <= max_s; ++sk) {
         const double rounds = (double)((wgs * sk + slots - 1) / slots);
-        const double cost = rounds / (double)sk + (double)(2 * sk + 1) * (double)ms / (double)g.K;
+        const double cost = rounds * (1.0 / (double)sk + fill) + (double)(2 * sk + 1) * per_slice;
         if (cost < 0.97 * best_cost) {
             best = (int)sk;
             best_cost = cost;

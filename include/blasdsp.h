@@ -53,7 +53,7 @@ BLASDSP_API int blasdsp_last_hip_error(void);
 /* ---- general entry points ----------------------------------------------- */
 
 /* Dispatches by shape: the MFMA tile kernel for general shapes, the
- * bandwidth-bound skinny kernel when M or N is at most 64 and the other side
+ * bandwidth-bound skinny kernel when M or N is at most 48 and the other side
  * is long enough for the skinny kernel's panels and K slices to fill the GPU;
  * a shape class recorded by blasdsp_autotune takes its recorded choice.
  * Replaces the per-mode __global__ functions of
