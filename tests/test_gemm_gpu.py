@@ -130,6 +130,20 @@ def test_skinny_splitk(orc, blas, dev, splitk):
         assert err <= (TOL64 if prec == "d" else tol32(1111)), (prec, err)
 
 
+# Short long sides take many K slices: with splitk 0 the library's picker
+# fills one round of workgroups (up to K / 512 slices, here 32 and 40), and
+# explicit counts up to 64 cut K into slices shorter than the 512 minimum.
+@pytest.mark.parametrize("case", [("d", "NN", 40, 700, 16384, 0), ("s", "TN", 700, 24, 16384, 0),
+                                  ("d", "NT", 24, 520, 16384, 64), ("s", "NN", 16, 300, 20480, 0),
+                                  ("s", "TT", 300, 48, 20480, 57)])
+def test_skinny_many_slices(orc, blas, dev, case):
+    prec, mode, M, N, K, splitk = case
+    got, ref = run_case(orc, blas, dev, prec, mode[0], mode[1], M, N, K, 1.25, -0.75, blas.ALGO_SKINNY,
+                        splitk=splitk, pad=4, cpad=4)
+    err, _ = orc.normwise(got, ref)
+    assert err <= (TOL64 if prec == "d" else tol32(K)), (case, err)
+
+
 @pytest.mark.parametrize("prec", ["d", "s"])
 @pytest.mark.parametrize("splitk", [1, 3])
 @pytest.mark.parametrize("case", [("NN", 32, 1100, 4000), ("TT", 1100, 32, 4000), ("NN", 28, 2100, 1000),
