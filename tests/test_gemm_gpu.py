@@ -70,7 +70,8 @@ def test_tile_kernel_modes(orc, blas, dev, prec, mode, shape):
 @pytest.mark.parametrize("mode", ["NN", "NT", "TN", "TT"])
 @pytest.mark.parametrize("shape", [(8, 300, 1000), (16, 64, 16), (32, 1000, 777), (40, 129, 64),
                                    (1, 70, 33), (300, 8, 1000), (1000, 32, 777), (129, 40, 64), (70, 1, 33),
-                                   (32, 100, 20), (300, 20, 5), (20, 1030, 1000), (1030, 20, 1000)])
+                                   (32, 100, 20), (300, 20, 5), (20, 1030, 1000), (1030, 20, 1000),
+                                   (48, 700, 1500), (1100, 33, 900)])
 @pytest.mark.parametrize("pad", [3, 4])
 def test_skinny_kernel_modes(orc, blas, dev, prec, mode, shape, pad):
     """Every mode against the oracle, with rows unaligned (pad 3: bounds-checked
