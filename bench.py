@@ -535,7 +535,7 @@ def bench_sgemm512(args):
     dC = torch.from_numpy(C0).cuda()
     stream = torch.cuda.current_stream()
     L = blas.lib()
-    call = lambda: blas.check("sgemm_NN", L.sgemm_NN(n, n, n, n, n, n, dA.data_ptr(), dB.data_ptr(), dC.data_ptr(),
+    call = lambda: blas.check("blasdsp_sgemm_NN", L.blasdsp_sgemm_NN(n, n, n, n, n, n, dA.data_ptr(), dB.data_ptr(), dC.data_ptr(),
                                                      1.0, 1.0, stream.cuda_stream))
     # parity on one call from the original C
     call()
@@ -639,7 +639,7 @@ def bench_sweep(args):
         for mode in MODES:
             a = A if mode[0] == "N" else AT
             b = B if mode[1] == "N" else BT
-            fn = getattr(L, "irr_thinB_sgemm_" + mode)
+            fn = getattr(L, "blasdsp_irr_thinB_sgemm_" + mode)
             lda, ldb = a.shape[1], b.shape[1]
             ms = _time_events(lambda: blas.check(mode, fn(lda, ldb, N, M, N, K, a.data_ptr(), b.data_ptr(),
                                                            C.data_ptr(), 1.0, 1.0, stream.cuda_stream)),

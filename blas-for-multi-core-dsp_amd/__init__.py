@@ -98,7 +98,7 @@ def header_functions() -> list[str]:
     """Names of every function declared in include/blasdsp.h."""
     text = HEADER.read_text()
     names = re.findall(r"^BLASDSP_API\s+[\w\s\*]*?\b(\w+)\s*\(", text, flags=re.M)
-    names += re.findall(r"^BLASDSP_DECLARE_IRR\((\w+),", text, flags=re.M)
+    names += re.findall(r"^BLASDSP_DECLARE_MODE\((\w+),", text, flags=re.M)
     return names
 
 
@@ -117,11 +117,18 @@ def _declare(L: ctypes.CDLL) -> None:
                                          ctypes.c_int, _vp]
     L.blasdsp_sgemm_ex.argtypes = gen + [_f, _vp, _i64, _vp, _i64, _f, _vp, _i64, ctypes.c_int,
                                          ctypes.c_int, _vp]
+    packed = [_i64, _i64, _i64, _vp, _vp, _vp]
     for name in header_functions():
-        if re.match(r"^(dgemm|irr_\w+_dgemm)_(NN|NT|TN|TT|null)$", name):
+        if re.match(r"^(dgemm|blasdsp_irr_\w+_dgemm)_(NN|NT|TN|TT|null)$", name):
             getattr(L, name).argtypes = mode_args + [_d, _d, _vp]
-        elif re.match(r"^(sgemm|irr_\w+_sgemm)_(NN|NT|TN|TT|null)$", name):
+        elif re.match(r"^blasdsp_(sgemm|irr_\w+_sgemm)_(NN|NT|TN|TT)$", name):
             getattr(L, name).argtypes = mode_args + [_f, _f, _vp]
+        elif re.match(r"^(sgemm_\w\w_qxx|sgemm_null|irr_thinB_sgemm_(NN|NT|TN|TT|null))$", name):
+            getattr(L, name).argtypes = packed + [_f, _f, _vp]
+        elif re.match(r"^irr_(shortA|thinB)_[ds]gemm_\w+$", name):
+            getattr(L, name).argtypes = packed + [_vp]
+    L.blasdsp_group_exec.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_uint64), _vp]
     L.blasdsp_fill_f64.argtypes = [_vp, _i64, _i64, _i64, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _d, _vp]
     L.blasdsp_fill_f32.argtypes = [_vp, _i64, _i64, _i64, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _f, _vp]
     L.blasdsp_sgemm_split_bf16.argtypes = gen + [_f, _vp, _i64, _vp, _i64, _f, _vp, _i64, ctypes.c_int, _vp]

@@ -125,56 +125,77 @@ bool skinny_shape(const Gemm<T> &g)
     return small <= 48 && large >= 2 * small && skinny_fits(g) && skinny_fills(g, num_cus());
 }
 
+}  // namespace
+
+template <typename T>
+int choose_kernel(const Gemm<T> &g, int algo, int splitk, KernelChoice &c)
+{
+    // 64 x 64 tiles when 128 x 128 tiles would leave most CUs idle
+    const int64_t t128 = ((g.M + 127) / 128) * ((g.N + 127) / 128);
+    c.tile = t128 < num_cus() ? 64 : 128;
+    if (algo == BLASDSP_ALGO_TILE) c.skinny = false;
+    else if (algo == BLASDSP_ALGO_SKINNY) c.skinny = skinny_fits(g);  // else: tile kernel
+    else if (algo == BLASDSP_ALGO_AUTO) {
+        TuneChoice t;
+        if (tune_lookup(sizeof(T) == 8, g.ta, g.tb, g.M, g.N, g.K, &t) &&
+            (t.algo == BLASDSP_ALGO_TILE || skinny_fits(g))) {
+            c.skinny = t.algo == BLASDSP_ALGO_SKINNY;
+            c.tile = t.tile;
+            if (splitk <= 0) splitk = t.splitk;
+        } else {
+            c.skinny = skinny_shape(g);
+        }
+    } else {
+        return BLASDSP_ERR_INVALID;
+    }
+    if (splitk <= 0) splitk = c.skinny ? skinny_pick_splitk(g, num_cus()) : tile_pick_splitk(g, c.tile, num_cus());
+    if (splitk > 256) return BLASDSP_ERR_INVALID;
+    c.splitk = splitk;
+    return BLASDSP_SUCCESS;
+}
+
+template <typename T>
+size_t kernel_scratch_bytes(const Gemm<T> &g, int splitk)
+{
+    return splitk > 1 ? (size_t)splitk * (size_t)g.M * (size_t)g.N * sizeof(T) : 0;
+}
+
+template <typename T>
+int launch_kernel(const Gemm<T> &g, const KernelChoice &c, T *work, hipStream_t s)
+{
+    if (c.skinny) return hip_status(launch_skinny(g, c.splitk, work, s));
+    return hip_status(launch_tile(g, c.tile, c.splitk, work, s));
+}
+
+template int choose_kernel<double>(const Gemm<double> &, int, int, KernelChoice &);
+template int choose_kernel<float>(const Gemm<float> &, int, int, KernelChoice &);
+template size_t kernel_scratch_bytes<double>(const Gemm<double> &, int);
+template size_t kernel_scratch_bytes<float>(const Gemm<float> &, int);
+template int launch_kernel<double>(const Gemm<double> &, const KernelChoice &, double *, hipStream_t);
+template int launch_kernel<float>(const Gemm<float> &, const KernelChoice &, float *, hipStream_t);
+
+namespace {
+
 template <typename T>
 int run(const Gemm<T> &g, int algo, int splitk, hipStream_t s)
 {
     if (g.M == 0 || g.N == 0) return BLASDSP_SUCCESS;
     if (g.K == 0 || g.alpha == T(0)) return hip_status(launch_scale(g, s));
-    // 64 x 64 tiles when 128 x 128 tiles would leave most CUs idle
-    const int64_t t128 = ((g.M + 127) / 128) * ((g.N + 127) / 128);
-    int tile = t128 < num_cus() ? 64 : 128;
-    bool use_skinny;
-    if (algo == BLASDSP_ALGO_TILE) use_skinny = false;
-    else if (algo == BLASDSP_ALGO_SKINNY) use_skinny = skinny_fits(g);  // else: tile kernel
-    else if (algo == BLASDSP_ALGO_AUTO) {
-        TuneChoice c;
-        if (tune_lookup(sizeof(T) == 8, g.ta, g.tb, g.M, g.N, g.K, &c) &&
-            (c.algo == BLASDSP_ALGO_TILE || skinny_fits(g))) {
-            use_skinny = c.algo == BLASDSP_ALGO_SKINNY;
-            tile = c.tile;
-            if (splitk <= 0) splitk = c.splitk;
-        } else {
-            use_skinny = skinny_shape(g);
-        }
-    } else {
-        return BLASDSP_ERR_INVALID;
-    }
-    if (!use_skinny) {
-        if (splitk <= 0) splitk = tile_pick_splitk(g, tile, num_cus());
-        if (splitk > 256) return BLASDSP_ERR_INVALID;
-        T *work = nullptr;
-        if (splitk > 1) {
-            const size_t bytes = (size_t)splitk * (size_t)g.M * (size_t)g.N * sizeof(T);
-            void *p = nullptr;
-            hipError_t e = workspace(s, bytes, &p);
-            if (e != hipSuccess) return hip_status(e);
-            work = (T *)p;
-        }
-        return hip_status(launch_tile(g, tile, splitk, work, s));
-    }
-
-    if (splitk <= 0) splitk = skinny_pick_splitk(g, num_cus());
-    if (splitk > 256) return BLASDSP_ERR_INVALID;
+    KernelChoice c;
+    int st = choose_kernel(g, algo, splitk, c);
+    if (st != BLASDSP_SUCCESS) return st;
     T *work = nullptr;
-    if (splitk > 1) {
-        const size_t bytes = (size_t)splitk * (size_t)g.M * (size_t)g.N * sizeof(T);
+    const size_t bytes = kernel_scratch_bytes(g, c.splitk);
+    if (bytes > 0) {
         void *p = nullptr;
         hipError_t e = workspace(s, bytes, &p);
         if (e != hipSuccess) return hip_status(e);
         work = (T *)p;
     }
-    return hip_status(launch_skinny(g, splitk, work, s));
+    return launch_kernel(g, c, work, s);
 }
+
+}  // namespace
 
 template <typename T>
 int gemm_entry(char ta, char tb, int64_t M, int64_t N, int64_t K, T alpha, const T *A, int64_t lda,
@@ -186,6 +207,11 @@ int gemm_entry(char ta, char tb, int64_t M, int64_t N, int64_t K, T alpha, const
     return run(g, algo, splitk, (hipStream_t)stream);
 }
 
+template int gemm_entry<double>(char, char, int64_t, int64_t, int64_t, double, const double *, int64_t,
+                                const double *, int64_t, double, double *, int64_t, int, int, void *);
+template int gemm_entry<float>(char, char, int64_t, int64_t, int64_t, float, const float *, int64_t,
+                               const float *, int64_t, float, float *, int64_t, int, int, void *);
+
 template <typename T>
 int null_entry(char ta, char tb, int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N,
                int64_t K, const T *A, const T *B, T *C, T alpha, T beta, void *stream)
@@ -196,7 +222,11 @@ int null_entry(char ta, char tb, int64_t lda, int64_t ldb, int64_t ldc, int64_t 
     return hip_status(launch_null((hipStream_t)stream));
 }
 
-}  // namespace
+template int null_entry<double>(char, char, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t, const double *,
+                                const double *, double *, double, double, void *);
+template int null_entry<float>(char, char, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t, const float *,
+                               const float *, float *, float, float, void *);
+
 }  // namespace blasdsp
 
 using blasdsp::gemm_entry;
@@ -263,26 +293,26 @@ BLASDSP_MODE_ENTRY(dgemm_NN, double, 'N', 'N', BLASDSP_ALGO_AUTO)
 BLASDSP_MODE_ENTRY(dgemm_NT, double, 'N', 'T', BLASDSP_ALGO_AUTO)
 BLASDSP_MODE_ENTRY(dgemm_TN, double, 'T', 'N', BLASDSP_ALGO_AUTO)
 BLASDSP_MODE_ENTRY(dgemm_TT, double, 'T', 'T', BLASDSP_ALGO_AUTO)
-BLASDSP_MODE_ENTRY(sgemm_NN, float, 'N', 'N', BLASDSP_ALGO_AUTO)
-BLASDSP_MODE_ENTRY(sgemm_NT, float, 'N', 'T', BLASDSP_ALGO_AUTO)
-BLASDSP_MODE_ENTRY(sgemm_TN, float, 'T', 'N', BLASDSP_ALGO_AUTO)
-BLASDSP_MODE_ENTRY(sgemm_TT, float, 'T', 'T', BLASDSP_ALGO_AUTO)
-BLASDSP_MODE_ENTRY(irr_shortA_dgemm_NN, double, 'N', 'N', BLASDSP_ALGO_SKINNY)
-BLASDSP_MODE_ENTRY(irr_shortA_dgemm_NT, double, 'N', 'T', BLASDSP_ALGO_SKINNY)
-BLASDSP_MODE_ENTRY(irr_shortA_dgemm_TN, double, 'T', 'N', BLASDSP_ALGO_SKINNY)
-BLASDSP_MODE_ENTRY(irr_shortA_dgemm_TT, double, 'T', 'T', BLASDSP_ALGO_SKINNY)
-BLASDSP_MODE_ENTRY(irr_shortA_sgemm_NN, float, 'N', 'N', BLASDSP_ALGO_SKINNY)
-BLASDSP_MODE_ENTRY(irr_shortA_sgemm_NT, float, 'N', 'T', BLASDSP_ALGO_SKINNY)
-BLASDSP_MODE_ENTRY(irr_shortA_sgemm_TN, float, 'T', 'N', BLASDSP_ALGO_SKINNY)
-BLASDSP_MODE_ENTRY(irr_shortA_sgemm_TT, float, 'T', 'T', BLASDSP_ALGO_SKINNY)
-BLASDSP_MODE_ENTRY(irr_thinB_dgemm_NN, double, 'N', 'N', BLASDSP_ALGO_SKINNY)
-BLASDSP_MODE_ENTRY(irr_thinB_dgemm_NT, double, 'N', 'T', BLASDSP_ALGO_SKINNY)
-BLASDSP_MODE_ENTRY(irr_thinB_dgemm_TN, double, 'T', 'N', BLASDSP_ALGO_SKINNY)
-BLASDSP_MODE_ENTRY(irr_thinB_dgemm_TT, double, 'T', 'T', BLASDSP_ALGO_SKINNY)
-BLASDSP_MODE_ENTRY(irr_thinB_sgemm_NN, float, 'N', 'N', BLASDSP_ALGO_SKINNY)
-BLASDSP_MODE_ENTRY(irr_thinB_sgemm_NT, float, 'N', 'T', BLASDSP_ALGO_SKINNY)
-BLASDSP_MODE_ENTRY(irr_thinB_sgemm_TN, float, 'T', 'N', BLASDSP_ALGO_SKINNY)
-BLASDSP_MODE_ENTRY(irr_thinB_sgemm_TT, float, 'T', 'T', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(blasdsp_sgemm_NN, float, 'N', 'N', BLASDSP_ALGO_AUTO)
+BLASDSP_MODE_ENTRY(blasdsp_sgemm_NT, float, 'N', 'T', BLASDSP_ALGO_AUTO)
+BLASDSP_MODE_ENTRY(blasdsp_sgemm_TN, float, 'T', 'N', BLASDSP_ALGO_AUTO)
+BLASDSP_MODE_ENTRY(blasdsp_sgemm_TT, float, 'T', 'T', BLASDSP_ALGO_AUTO)
+BLASDSP_MODE_ENTRY(blasdsp_irr_shortA_dgemm_NN, double, 'N', 'N', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(blasdsp_irr_shortA_dgemm_NT, double, 'N', 'T', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(blasdsp_irr_shortA_dgemm_TN, double, 'T', 'N', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(blasdsp_irr_shortA_dgemm_TT, double, 'T', 'T', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(blasdsp_irr_shortA_sgemm_NN, float, 'N', 'N', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(blasdsp_irr_shortA_sgemm_NT, float, 'N', 'T', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(blasdsp_irr_shortA_sgemm_TN, float, 'T', 'N', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(blasdsp_irr_shortA_sgemm_TT, float, 'T', 'T', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(blasdsp_irr_thinB_dgemm_NN, double, 'N', 'N', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(blasdsp_irr_thinB_dgemm_NT, double, 'N', 'T', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(blasdsp_irr_thinB_dgemm_TN, double, 'T', 'N', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(blasdsp_irr_thinB_dgemm_TT, double, 'T', 'T', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(blasdsp_irr_thinB_sgemm_NN, float, 'N', 'N', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(blasdsp_irr_thinB_sgemm_NT, float, 'N', 'T', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(blasdsp_irr_thinB_sgemm_TN, float, 'T', 'N', BLASDSP_ALGO_SKINNY)
+BLASDSP_MODE_ENTRY(blasdsp_irr_thinB_sgemm_TT, float, 'T', 'T', BLASDSP_ALGO_SKINNY)
 #undef BLASDSP_MODE_ENTRY
 
 int dgemm_null(int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N, int64_t K,
@@ -291,11 +321,6 @@ int dgemm_null(int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N, int6
     return blasdsp::null_entry<double>('N', 'N', lda, ldb, ldc, M, N, K, A, B, C, alpha, beta, stream);
 }
 
-int sgemm_null(int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N, int64_t K,
-               const float *A, const float *B, float *C, float alpha, float beta, void *stream)
-{
-    return blasdsp::null_entry<float>('N', 'N', lda, ldb, ldc, M, N, K, A, B, C, alpha, beta, stream);
-}
 
 }  // extern "C"
 

@@ -77,11 +77,23 @@ typedef int (*gemm_fn)(int64_t, int64_t, int64_t, int64_t, int64_t, int64_t, con
                     (float)be, s);                                                                      \
     }
 ADAPT_D(dgemm_NN) ADAPT_D(dgemm_NT) ADAPT_D(dgemm_TN) ADAPT_D(dgemm_TT) ADAPT_D(dgemm_null)
-ADAPT_S(sgemm_NN) ADAPT_S(sgemm_NT) ADAPT_S(sgemm_TN) ADAPT_S(sgemm_TT) ADAPT_S(sgemm_null)
-ADAPT_D(irr_shortA_dgemm_NN) ADAPT_D(irr_shortA_dgemm_NT) ADAPT_D(irr_shortA_dgemm_TN) ADAPT_D(irr_shortA_dgemm_TT)
-ADAPT_S(irr_shortA_sgemm_NN) ADAPT_S(irr_shortA_sgemm_NT) ADAPT_S(irr_shortA_sgemm_TN) ADAPT_S(irr_shortA_sgemm_TT)
-ADAPT_D(irr_thinB_dgemm_NN) ADAPT_D(irr_thinB_dgemm_NT) ADAPT_D(irr_thinB_dgemm_TN) ADAPT_D(irr_thinB_dgemm_TT)
-ADAPT_S(irr_thinB_sgemm_NN) ADAPT_S(irr_thinB_sgemm_NT) ADAPT_S(irr_thinB_sgemm_TN) ADAPT_S(irr_thinB_sgemm_TT)
+ADAPT_S(blasdsp_sgemm_NN) ADAPT_S(blasdsp_sgemm_NT) ADAPT_S(blasdsp_sgemm_TN) ADAPT_S(blasdsp_sgemm_TT)
+ADAPT_D(blasdsp_irr_shortA_dgemm_NN) ADAPT_D(blasdsp_irr_shortA_dgemm_NT) ADAPT_D(blasdsp_irr_shortA_dgemm_TN)
+ADAPT_D(blasdsp_irr_shortA_dgemm_TT)
+ADAPT_S(blasdsp_irr_shortA_sgemm_NN) ADAPT_S(blasdsp_irr_shortA_sgemm_NT) ADAPT_S(blasdsp_irr_shortA_sgemm_TN)
+ADAPT_S(blasdsp_irr_shortA_sgemm_TT)
+ADAPT_D(blasdsp_irr_thinB_dgemm_NN) ADAPT_D(blasdsp_irr_thinB_dgemm_NT) ADAPT_D(blasdsp_irr_thinB_dgemm_TN)
+ADAPT_D(blasdsp_irr_thinB_dgemm_TT)
+ADAPT_S(blasdsp_irr_thinB_sgemm_NN) ADAPT_S(blasdsp_irr_thinB_sgemm_NT) ADAPT_S(blasdsp_irr_thinB_sgemm_TN)
+ADAPT_S(blasdsp_irr_thinB_sgemm_TT)
+
+/* sgemm_null takes the reference's packed list (M, N, K, A, B, C, alpha, beta) */
+static int a_sgemm_null(int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N, int64_t K, const void *A,
+                        const void *B, void *C, double al, double be, void *s)
+{
+    (void)lda; (void)ldb; (void)ldc;
+    return sgemm_null(M, N, K, (const float *)A, (const float *)B, (float *)C, (float)al, (float)be, s);
+}
 
 struct family {
     const char *name;
@@ -92,11 +104,15 @@ struct family {
 
 static const struct family FAMILIES[] = {
     {"dgemm", 1, {a_dgemm_NN, a_dgemm_NT, a_dgemm_TN, a_dgemm_TT}, a_dgemm_null},
-    {"sgemm", 0, {a_sgemm_NN, a_sgemm_NT, a_sgemm_TN, a_sgemm_TT}, a_sgemm_null},
-    {"irr_shortA_dgemm", 1, {a_irr_shortA_dgemm_NN, a_irr_shortA_dgemm_NT, a_irr_shortA_dgemm_TN, a_irr_shortA_dgemm_TT}, a_dgemm_null},
-    {"irr_shortA_sgemm", 0, {a_irr_shortA_sgemm_NN, a_irr_shortA_sgemm_NT, a_irr_shortA_sgemm_TN, a_irr_shortA_sgemm_TT}, a_sgemm_null},
-    {"irr_thinB_dgemm", 1, {a_irr_thinB_dgemm_NN, a_irr_thinB_dgemm_NT, a_irr_thinB_dgemm_TN, a_irr_thinB_dgemm_TT}, a_dgemm_null},
-    {"irr_thinB_sgemm", 0, {a_irr_thinB_sgemm_NN, a_irr_thinB_sgemm_NT, a_irr_thinB_sgemm_TN, a_irr_thinB_sgemm_TT}, a_sgemm_null},
+    {"sgemm", 0, {a_blasdsp_sgemm_NN, a_blasdsp_sgemm_NT, a_blasdsp_sgemm_TN, a_blasdsp_sgemm_TT}, a_sgemm_null},
+    {"irr_shortA_dgemm", 1, {a_blasdsp_irr_shortA_dgemm_NN, a_blasdsp_irr_shortA_dgemm_NT,
+                             a_blasdsp_irr_shortA_dgemm_TN, a_blasdsp_irr_shortA_dgemm_TT}, a_dgemm_null},
+    {"irr_shortA_sgemm", 0, {a_blasdsp_irr_shortA_sgemm_NN, a_blasdsp_irr_shortA_sgemm_NT,
+                             a_blasdsp_irr_shortA_sgemm_TN, a_blasdsp_irr_shortA_sgemm_TT}, a_sgemm_null},
+    {"irr_thinB_dgemm", 1, {a_blasdsp_irr_thinB_dgemm_NN, a_blasdsp_irr_thinB_dgemm_NT,
+                            a_blasdsp_irr_thinB_dgemm_TN, a_blasdsp_irr_thinB_dgemm_TT}, a_dgemm_null},
+    {"irr_thinB_sgemm", 0, {a_blasdsp_irr_thinB_sgemm_NN, a_blasdsp_irr_thinB_sgemm_NT,
+                            a_blasdsp_irr_thinB_sgemm_TN, a_blasdsp_irr_thinB_sgemm_TT}, a_sgemm_null},
 };
 
 static void fill(int f64, void *X, int64_t rows, int64_t cols, int64_t ld, int trans, uint64_t seed)

@@ -18,10 +18,13 @@
  * Calls on distinct streams may run concurrently.
  *
  * Semantics relative to the reference.  The reference's dgemm and irr kernels
- * compute C += op(A)op(B) (here: alpha = 1, beta = 1); sgemm V1.3.2 and
- * irr_thinB_sgemm V1.2 compute C = alpha*op(A)op(B) + beta with a SCALAR beta
- * and ignore the input C.  This library implements the BLAS form above for
- * every entry point; the mapping is written out beside each declaration.
+ * compute C += op(A)op(B); sgemm V1.3.2 and irr_thinB_sgemm V1.2 compute
+ * C = alpha*op(A)op(B) + beta with a SCALAR beta and ignore the input C.  The
+ * entry points that carry the reference's packed-operand names and argument
+ * lists (sgemm_*_qxx, irr_*) keep those semantics, so they are drop-ins;
+ * dgemm_NN..TT keep the reference's leading-dimension argument list with
+ * alpha and beta appended (1, 1 reproduce C += op(A)op(B)); every other entry
+ * point implements the BLAS form above.
  */
 #ifndef BLASDSP_H
 #define BLASDSP_H
@@ -70,7 +73,7 @@ BLASDSP_API int blasdsp_sgemm(char transA, char transB, int64_t M, int64_t N, in
                   const float *B, int64_t ldb, float beta,
                   float *C, int64_t ldc, void *stream);
 
-/* ---- per-mode entry points (reference names) ----------------------------
+/* ---- dgemm V1.1 per-mode entry points (reference names) ------------------
  * Argument order follows the reference (lda, ldb, ldc, M, N, K, A, B, C) with
  * the barrier ids dropped (no group barriers exist on the GPU) and alpha,
  * beta, stream appended (alpha/beta last, as in sgemm_NN_qxx).
@@ -101,59 +104,137 @@ BLASDSP_API int dgemm_null(int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int
                const double *A, const double *B, double *C, double alpha, double beta,
                void *stream);
 
-/* sgemm_release_V1.3.2/include/sgemm_kernel.h:11-26 (sgemm_*_qxx, packed
- * lda = K, ldb = N, ldc = N there; explicit leading dimensions here).
- * Reference C = alpha*op(A)op(B) + beta(scalar) with C ignored corresponds to
- * beta = 0 here followed by adding the scalar. */
-BLASDSP_API int sgemm_NN(int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N, int64_t K,
-             const float *A, const float *B, float *C, float alpha, float beta,
-             void *stream);
-BLASDSP_API int sgemm_NT(int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N, int64_t K,
-             const float *A, const float *B, float *C, float alpha, float beta,
-             void *stream);
-BLASDSP_API int sgemm_TN(int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N, int64_t K,
-             const float *A, const float *B, float *C, float alpha, float beta,
-             void *stream);
-BLASDSP_API int sgemm_TT(int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N, int64_t K,
-             const float *A, const float *B, float *C, float alpha, float beta,
-             void *stream);
-/* sgemm_release_V1.3.2/include/sgemm_kernel.h:25 */
-BLASDSP_API int sgemm_null(int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N, int64_t K,
-               const float *A, const float *B, float *C, float alpha, float beta,
-               void *stream);
+/* ---- the reference's packed-operand kernels ------------------------------
+ * The reference's names, argument order and semantics, with the barrier ids
+ * (b_id; bid_main, bid_T) dropped and a stream appended.  Operands are
+ * packed as in the reference's demos: A is M x K (lda = K) or, for the T
+ * modes, K x M (lda = M); B is K x N (ldb = N) or N x K (ldb = K); C is
+ * M x N (ldc = N).  The ld-taking BLAS-semantics generalisations are the
+ * blasdsp_* entries below.
+ *
+ * sgemm V1.3.2: C = alpha op(A) op(B) + beta, beta a SCALAR added to every
+ * element, the input C ignored (sgemm_release_V1.3.2/device_code/
+ * sgemm_kernel.c:2681,2745-2749).  Implemented as C = beta (one write pass
+ * over C, the reference's C_plus_beta, sgemm_kernel.c:25-160), then the BLAS
+ * GEMM with beta = 1. */
+/* sgemm_release_V1.3.2/include/sgemm_kernel.h:11 */
+BLASDSP_API int sgemm_NN_qxx(int64_t M, int64_t N, int64_t K, const float *A, const float *B, float *C,
+                             float alpha, float beta, void *stream);
+/* sgemm_release_V1.3.2/include/sgemm_kernel.h:15 */
+BLASDSP_API int sgemm_NT_qxx(int64_t M, int64_t N, int64_t K, const float *A, const float *B, float *C,
+                             float alpha, float beta, void *stream);
+/* sgemm_release_V1.3.2/include/sgemm_kernel.h:18 (bid_main, bid_T dropped) */
+BLASDSP_API int sgemm_TN_qxx(int64_t M, int64_t N, int64_t K, const float *A, const float *B, float *C,
+                             float alpha, float beta, void *stream);
+/* sgemm_release_V1.3.2/include/sgemm_kernel.h:22 */
+BLASDSP_API int sgemm_TT_qxx(int64_t M, int64_t N, int64_t K, const float *A, const float *B, float *C,
+                             float alpha, float beta, void *stream);
+/* sgemm_release_V1.3.2/include/sgemm_kernel.h:25 — empty kernel launch;
+ * arguments are checked like sgemm_NN_qxx's and otherwise ignored. */
+BLASDSP_API int sgemm_null(int64_t M, int64_t N, int64_t K, const float *A, const float *B, float *C,
+                           float alpha, float beta, void *stream);
 
-/* ---- irregular shapes: shortA (small M) and thinB (small N) --------------
- * Same arguments as dgemm_NN.  These always run the bandwidth-bound skinny
- * kernel (op(B) or op(A) streamed once from HBM, the small operand held on
- * chip, split-K with a deterministic reduction when the long side alone does
- * not fill the GPU).  Any shape is accepted; they pay off for M <= 64
- * (shortA) or N <= 64 (thinB).
- *   irr_shortA_dgemm_release_V0.0/include/irr_shortA_dgemm_kernel.h:11-23
- *     irr_shortA_dgemm_{NN,NT,TN,TT}_dbuf(M, N, K, b_id, A, B, C)   C += op(A)op(B)
- *   irr_shortA_sgemm_release_V0.0/include/irr_shortA_sgemm_kernel.h:11-23
- *   irr_thinB_dgemm_release_V0.0/include/irr_thinB_dgemm_kernel.h:11-23
- *   irr_thinB_sgemm_release_V1.2/include/irr_thinB_sgemm_kernel.h:11-23
- *     irr_thinB_sgemm_{NN,NT,TN,TT}(M, N, K, b_id, A, B, C, alpha, beta)  */
-#define BLASDSP_DECLARE_IRR(name, T)                                                    \
+/* irr_shortA (small M) and irr_thinB (small N): C += op(A) op(B), except
+ * irr_thinB_sgemm V1.2, which computes C = alpha op(A) op(B) + beta (scalar
+ * beta, input C ignored; irr_thinB_sgemm_release_V1.2/device_code/
+ * irr_thinB_sgemm_kernel.c:3095-3112,3182-3193), done as for sgemm_*_qxx.
+ * They run the bandwidth-bound skinny kernel (op(B) or op(A) streamed once
+ * from HBM, the small operand held on chip, split-K with a deterministic
+ * reduction when the long side alone does not fill the GPU); shapes it does
+ * not take (small side above 64, or per-lane load offsets at or past 2^31
+ * bytes) run on the tile kernel.  The *_null entries launch the empty kernel
+ * after checking their arguments. */
+/* irr_shortA_dgemm_release_V0.0/include/irr_shortA_dgemm_kernel.h:11,15,18,22,25 */
+BLASDSP_API int irr_shortA_dgemm_NN_dbuf(int64_t M, int64_t N, int64_t K, const double *A, const double *B,
+                                         double *C, void *stream);
+BLASDSP_API int irr_shortA_dgemm_NT_dbuf(int64_t M, int64_t N, int64_t K, const double *A, const double *B,
+                                         double *C, void *stream);
+BLASDSP_API int irr_shortA_dgemm_TN_dbuf(int64_t M, int64_t N, int64_t K, const double *A, const double *B,
+                                         double *C, void *stream);
+BLASDSP_API int irr_shortA_dgemm_TT_dbuf(int64_t M, int64_t N, int64_t K, const double *A, const double *B,
+                                         double *C, void *stream);
+BLASDSP_API int irr_shortA_dgemm_null(int64_t M, int64_t N, int64_t K, const double *A, const double *B,
+                                      double *C, void *stream);
+/* irr_shortA_sgemm_release_V0.0/include/irr_shortA_sgemm_kernel.h:11,15,18,22,25 */
+BLASDSP_API int irr_shortA_sgemm_NN(int64_t M, int64_t N, int64_t K, const float *A, const float *B, float *C,
+                                    void *stream);
+BLASDSP_API int irr_shortA_sgemm_NT(int64_t M, int64_t N, int64_t K, const float *A, const float *B, float *C,
+                                    void *stream);
+BLASDSP_API int irr_shortA_sgemm_TN(int64_t M, int64_t N, int64_t K, const float *A, const float *B, float *C,
+                                    void *stream);
+BLASDSP_API int irr_shortA_sgemm_TT(int64_t M, int64_t N, int64_t K, const float *A, const float *B, float *C,
+                                    void *stream);
+BLASDSP_API int irr_shortA_sgemm_null(int64_t M, int64_t N, int64_t K, const float *A, const float *B, float *C,
+                                      void *stream);
+/* irr_thinB_dgemm_release_V0.0/include/irr_thinB_dgemm_kernel.h:11-25 (the
+ * header's comments on the TT and NN prototypes are swapped; the names are
+ * what the kernels compute) */
+BLASDSP_API int irr_thinB_dgemm_NN_dbuf(int64_t M, int64_t N, int64_t K, const double *A, const double *B,
+                                        double *C, void *stream);
+BLASDSP_API int irr_thinB_dgemm_NT_dbuf(int64_t M, int64_t N, int64_t K, const double *A, const double *B,
+                                        double *C, void *stream);
+BLASDSP_API int irr_thinB_dgemm_TN_dbuf(int64_t M, int64_t N, int64_t K, const double *A, const double *B,
+                                        double *C, void *stream);
+BLASDSP_API int irr_thinB_dgemm_TT_dbuf(int64_t M, int64_t N, int64_t K, const double *A, const double *B,
+                                        double *C, void *stream);
+BLASDSP_API int irr_thinB_dgemm_null(int64_t M, int64_t N, int64_t K, const double *A, const double *B,
+                                     double *C, void *stream);
+/* irr_thinB_sgemm_release_V1.2/include/irr_thinB_sgemm_kernel.h:11-25 */
+BLASDSP_API int irr_thinB_sgemm_NN(int64_t M, int64_t N, int64_t K, const float *A, const float *B, float *C,
+                                   float alpha, float beta, void *stream);
+BLASDSP_API int irr_thinB_sgemm_NT(int64_t M, int64_t N, int64_t K, const float *A, const float *B, float *C,
+                                   float alpha, float beta, void *stream);
+BLASDSP_API int irr_thinB_sgemm_TN(int64_t M, int64_t N, int64_t K, const float *A, const float *B, float *C,
+                                   float alpha, float beta, void *stream);
+BLASDSP_API int irr_thinB_sgemm_TT(int64_t M, int64_t N, int64_t K, const float *A, const float *B, float *C,
+                                   float alpha, float beta, void *stream);
+BLASDSP_API int irr_thinB_sgemm_null(int64_t M, int64_t N, int64_t K, const float *A, const float *B, float *C,
+                                     float alpha, float beta, void *stream);
+
+/* ---- by-name launch -------------------------------------------------------
+ * Replaces the reference host's hthread_group_exec(tg_id, name, n_scalar,
+ * n_ptr, args) (dgemm_release_V1.1/host_code/dgemm_demo.c:151): runs the
+ * launcher `name` (launch_dgemm_NN ... launch_irr_thinB_sgemm_null, the names
+ * of the reference's *_launcher.c files) on the same argument words —
+ * n_scalar 64-bit scalars (dimensions, then barrier ids, which are ignored)
+ * followed by n_ptr = 3 device pointers A, B, C.  The counts must be the
+ * reference launcher's (7 for launch_dgemm_*, 8 for launch_dgemm_TN; 4 for the
+ * packed families, 5 for their two-barrier modes).  The sgemm and
+ * irr_thinB_sgemm launchers use the reference launchers' compiled-in
+ * alpha = 5, beta = 3 (sgemm_release_V1.3.2/device_code/sgemm_launcher.c:11-12).
+ * Asynchronous on `stream`.  BLASDSP_ERR_UNSUPPORTED for an unknown name. */
+BLASDSP_API int blasdsp_group_exec(const char *name, int n_scalar, int n_ptr, const uint64_t *args,
+                                   void *stream);
+
+/* ---- per-mode BLAS entries with leading dimensions -------------------------
+ * Generalisations of the packed kernels above: BLAS semantics
+ * C = alpha op(A) op(B) + beta C and explicit leading dimensions, in dgemm_NN's
+ * argument order.  blasdsp_sgemm_* run the AUTO dispatch (as blasdsp_sgemm);
+ * blasdsp_irr_* the skinny kernel (the tile kernel where it does not fit). */
+#define BLASDSP_DECLARE_MODE(name, T)                                                   \
     BLASDSP_API int name(int64_t lda, int64_t ldb, int64_t ldc, int64_t M, int64_t N, int64_t K,    \
              const T *A, const T *B, T *C, T alpha, T beta, void *stream);
-BLASDSP_DECLARE_IRR(irr_shortA_dgemm_NN, double)
-BLASDSP_DECLARE_IRR(irr_shortA_dgemm_NT, double)
-BLASDSP_DECLARE_IRR(irr_shortA_dgemm_TN, double)
-BLASDSP_DECLARE_IRR(irr_shortA_dgemm_TT, double)
-BLASDSP_DECLARE_IRR(irr_shortA_sgemm_NN, float)
-BLASDSP_DECLARE_IRR(irr_shortA_sgemm_NT, float)
-BLASDSP_DECLARE_IRR(irr_shortA_sgemm_TN, float)
-BLASDSP_DECLARE_IRR(irr_shortA_sgemm_TT, float)
-BLASDSP_DECLARE_IRR(irr_thinB_dgemm_NN, double)
-BLASDSP_DECLARE_IRR(irr_thinB_dgemm_NT, double)
-BLASDSP_DECLARE_IRR(irr_thinB_dgemm_TN, double)
-BLASDSP_DECLARE_IRR(irr_thinB_dgemm_TT, double)
-BLASDSP_DECLARE_IRR(irr_thinB_sgemm_NN, float)
-BLASDSP_DECLARE_IRR(irr_thinB_sgemm_NT, float)
-BLASDSP_DECLARE_IRR(irr_thinB_sgemm_TN, float)
-BLASDSP_DECLARE_IRR(irr_thinB_sgemm_TT, float)
-#undef BLASDSP_DECLARE_IRR
+BLASDSP_DECLARE_MODE(blasdsp_sgemm_NN, float)
+BLASDSP_DECLARE_MODE(blasdsp_sgemm_NT, float)
+BLASDSP_DECLARE_MODE(blasdsp_sgemm_TN, float)
+BLASDSP_DECLARE_MODE(blasdsp_sgemm_TT, float)
+BLASDSP_DECLARE_MODE(blasdsp_irr_shortA_dgemm_NN, double)
+BLASDSP_DECLARE_MODE(blasdsp_irr_shortA_dgemm_NT, double)
+BLASDSP_DECLARE_MODE(blasdsp_irr_shortA_dgemm_TN, double)
+BLASDSP_DECLARE_MODE(blasdsp_irr_shortA_dgemm_TT, double)
+BLASDSP_DECLARE_MODE(blasdsp_irr_shortA_sgemm_NN, float)
+BLASDSP_DECLARE_MODE(blasdsp_irr_shortA_sgemm_NT, float)
+BLASDSP_DECLARE_MODE(blasdsp_irr_shortA_sgemm_TN, float)
+BLASDSP_DECLARE_MODE(blasdsp_irr_shortA_sgemm_TT, float)
+BLASDSP_DECLARE_MODE(blasdsp_irr_thinB_dgemm_NN, double)
+BLASDSP_DECLARE_MODE(blasdsp_irr_thinB_dgemm_NT, double)
+BLASDSP_DECLARE_MODE(blasdsp_irr_thinB_dgemm_TN, double)
+BLASDSP_DECLARE_MODE(blasdsp_irr_thinB_dgemm_TT, double)
+BLASDSP_DECLARE_MODE(blasdsp_irr_thinB_sgemm_NN, float)
+BLASDSP_DECLARE_MODE(blasdsp_irr_thinB_sgemm_NT, float)
+BLASDSP_DECLARE_MODE(blasdsp_irr_thinB_sgemm_TN, float)
+BLASDSP_DECLARE_MODE(blasdsp_irr_thinB_sgemm_TT, float)
+#undef BLASDSP_DECLARE_MODE
 
 /* ---- kernel selection (explicit) ----------------------------------------- */
 enum blasdsp_algo {
@@ -178,6 +259,7 @@ BLASDSP_API int blasdsp_sgemm_ex(char transA, char transB, int64_t M, int64_t N,
                      float alpha, const float *A, int64_t lda,
                      const float *B, int64_t ldb, float beta,
                      float *C, int64_t ldc, int algo, int splitk, void *stream);
+
 
 /* ---- precision forms (no counterpart in the reference) -------------------
  * FP32 GEMM on the BF16 matrix cores: each FP32 operand is split into BF16
@@ -245,6 +327,7 @@ BLASDSP_API int blasdsp_group_plan_create(int max_problems, blasdsp_group_plan *
 BLASDSP_API int blasdsp_group_plan_run(blasdsp_group_plan *plan, const blasdsp_gemm_desc *descs,
                                        int count, void *stream);
 BLASDSP_API int blasdsp_group_plan_destroy(blasdsp_group_plan *plan);
+
 
 /* ---- continuous-batching request loop (no counterpart in the reference) ----
  * blasdsp_serve() plays a list of GEMM requests against the GPU as a server

@@ -200,6 +200,186 @@ static void check_grouped(void)
     report("blasdsp_gemm_grouped (6)", bad ? 1.0 : worst, tol32(512), st);
 }
 
+/* ---- the reference's packed-operand kernels, called as the reference's demos
+ * call them: packed operands (A M x K or K x M, B K x N or N x K, C M x N),
+ * dimensions first (irr_shortA_dgemm_release_V0.0/host_code/
+ * irr_shortA_dgemm_demo.c:50-120, sgemm_release_V1.3.2/host_code/
+ * sgemm_demo.c:82-144).  The scalar-beta forms get C pre-filled with 999, as
+ * the reference's sgemm demo does to show that the input C is ignored. */
+enum { SEM_ACC = 0, SEM_SCALAR = 1 };
+typedef int (*packed_d)(int64_t, int64_t, int64_t, const double *, const double *, double *, void *);
+typedef int (*packed_s)(int64_t, int64_t, int64_t, const float *, const float *, float *, void *);
+typedef int (*packed_sab)(int64_t, int64_t, int64_t, const float *, const float *, float *, float, float, void *);
+
+/* Device copies of packed op(A), op(B) and C for one case, with the oracle's
+ * expected result R (M x N, FP64). */
+struct packed_case {
+    int f64;
+    char ta, tb;
+    int64_t M, N, K;
+    void *dA, *dB, *dC, *hC;
+    double *R;
+};
+
+static void packed_setup(struct packed_case *c, int f64, char ta, char tb, int64_t M, int64_t N, int64_t K,
+                         int sem, double alpha, double beta, int dist, uint64_t seed)
+{
+    int64_t ar, ac, br, bc;
+    stored(ta, M, K, &ar, &ac);
+    stored(tb, K, N, &br, &bc);
+    const size_t s = f64 ? 8 : 4;
+    void *A = malloc(s * ar * ac), *B = malloc(s * br * bc), *C = malloc(s * M * N);
+    c->f64 = f64; c->ta = ta; c->tb = tb; c->M = M; c->N = N; c->K = K;
+    c->R = malloc(sizeof(double) * M * N);
+    const int distB = dist == ORACLE_PATTERN11 ? ORACLE_PATTERN7 : dist;
+    if (f64) {
+        oracle_fill_f64(A, M, K, ac, ta == 'T', dist, seed, 0.0);
+        oracle_fill_f64(B, K, N, bc, tb == 'T', distB, seed + 1, 0.0);
+        oracle_fill_f64(C, M, N, N, 0, dist, seed + 2, 0.0);
+        memcpy(c->R, C, 8 * M * N);
+        oracle_gemm_xp_f64(ta, tb, M, N, K, alpha, A, ac, B, bc, beta, c->R, N);
+    } else {
+        oracle_fill_f32(A, M, K, ac, ta == 'T', dist, seed, 0.0f);
+        oracle_fill_f32(B, K, N, bc, tb == 'T', distB, seed + 1, 0.0f);
+        oracle_fill_f32(C, M, N, N, 0, dist, seed + 2, 0.0f);
+        if (sem == SEM_SCALAR) {
+            /* alpha op(A)op(B) + beta = the BLAS form on a C of ones */
+            float *ones = malloc(4 * M * N);
+            for (int64_t i = 0; i < M * N; ++i) ones[i] = 1.0f;
+            oracle_gemm_xp_f32(ta, tb, M, N, K, alpha, A, ac, B, bc, beta, ones, N, c->R, N);
+            for (int64_t i = 0; i < M * N; ++i) ((float *)C)[i] = 999.0f;
+            free(ones);
+        } else {
+            oracle_gemm_xp_f32(ta, tb, M, N, K, alpha, A, ac, B, bc, beta, C, N, c->R, N);
+        }
+    }
+    HIP_OK(hipMalloc(&c->dA, s * ar * ac));
+    HIP_OK(hipMalloc(&c->dB, s * br * bc));
+    HIP_OK(hipMalloc(&c->dC, s * M * N));
+    HIP_OK(hipMemcpy(c->dA, A, s * ar * ac, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->dB, B, s * br * bc, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->dC, C, s * M * N, hipMemcpyHostToDevice));
+    c->hC = C;
+    free(A); free(B);
+}
+
+/* err: normwise against R, or for integer patterns 0 / 1 for exact / not */
+static double packed_finish(struct packed_case *c, int exact)
+{
+    HIP_OK(hipDeviceSynchronize());
+    const size_t s = c->f64 ? 8 : 4;
+    HIP_OK(hipMemcpy(c->hC, c->dC, s * c->M * c->N, hipMemcpyDeviceToHost));
+    double maxabs = 0.0, err;
+    if (exact) {
+        err = 0.0;
+        for (int64_t i = 0; i < c->M * c->N; ++i) {
+            const double v = c->f64 ? ((double *)c->hC)[i] : (double)((float *)c->hC)[i];
+            if (v != c->R[i]) err = 1.0;
+        }
+    } else {
+        err = c->f64 ? oracle_normwise_f64(c->M, c->N, c->hC, c->N, c->R, c->N, &maxabs)
+                     : oracle_normwise_f32(c->M, c->N, c->hC, c->N, c->R, c->N, &maxabs);
+    }
+    HIP_OK(hipFree(c->dA));
+    HIP_OK(hipFree(c->dB));
+    HIP_OK(hipFree(c->dC));
+    free(c->hC); free(c->R);
+    return err;
+}
+
+static void check_packed(void)
+{
+    const char *mname[4] = {"NN", "NT", "TN", "TT"};
+    const packed_d sa_d[4] = {irr_shortA_dgemm_NN_dbuf, irr_shortA_dgemm_NT_dbuf, irr_shortA_dgemm_TN_dbuf,
+                              irr_shortA_dgemm_TT_dbuf};
+    const packed_s sa_s[4] = {irr_shortA_sgemm_NN, irr_shortA_sgemm_NT, irr_shortA_sgemm_TN, irr_shortA_sgemm_TT};
+    const packed_d tb_d[4] = {irr_thinB_dgemm_NN_dbuf, irr_thinB_dgemm_NT_dbuf, irr_thinB_dgemm_TN_dbuf,
+                              irr_thinB_dgemm_TT_dbuf};
+    const packed_sab tb_s[4] = {irr_thinB_sgemm_NN, irr_thinB_sgemm_NT, irr_thinB_sgemm_TN, irr_thinB_sgemm_TT};
+    const packed_sab qxx[4] = {sgemm_NN_qxx, sgemm_NT_qxx, sgemm_TN_qxx, sgemm_TT_qxx};
+    char name[64];
+    struct packed_case c;
+    for (int m = 0; m < 4; ++m) {
+        const char ta = mname[m][0], tb = mname[m][1];
+        /* random operands vs the oracle */
+        packed_setup(&c, 1, ta, tb, 24, 3000, 2048, SEM_ACC, 1.0, 1.0, ORACLE_UNIFORM, 100 + m);
+        int st = sa_d[m](c.M, c.N, c.K, c.dA, c.dB, c.dC, NULL);
+        snprintf(name, sizeof name, "irr_shortA_dgemm_%s_dbuf", mname[m]);
+        report(name, packed_finish(&c, 0), 1e-13, st);
+
+        packed_setup(&c, 0, ta, tb, 32, 2500, 1500, SEM_ACC, 1.0, 1.0, ORACLE_UNIFORM, 110 + m);
+        st = sa_s[m](c.M, c.N, c.K, c.dA, c.dB, c.dC, NULL);
+        snprintf(name, sizeof name, "irr_shortA_sgemm_%s", mname[m]);
+        report(name, packed_finish(&c, 0), tol32(1500), st);
+
+        packed_setup(&c, 1, ta, tb, 2800, 16, 1800, SEM_ACC, 1.0, 1.0, ORACLE_UNIFORM, 120 + m);
+        st = tb_d[m](c.M, c.N, c.K, c.dA, c.dB, c.dC, NULL);
+        snprintf(name, sizeof name, "irr_thinB_dgemm_%s_dbuf", mname[m]);
+        report(name, packed_finish(&c, 0), 1e-13, st);
+
+        packed_setup(&c, 0, ta, tb, 3000, 24, 3000, SEM_SCALAR, 1.25, -0.5, ORACLE_UNIFORM, 130 + m);
+        st = tb_s[m](c.M, c.N, c.K, c.dA, c.dB, c.dC, 1.25f, -0.5f, NULL);
+        snprintf(name, sizeof name, "irr_thinB_sgemm_%s (C=999 in)", mname[m]);
+        report(name, packed_finish(&c, 0), tol32(3000), st);
+
+        packed_setup(&c, 0, ta, tb, 257, 129, 1000, SEM_SCALAR, 1.5, 3.0, ORACLE_UNIFORM, 140 + m);
+        st = qxx[m](c.M, c.N, c.K, c.dA, c.dB, c.dC, 1.5f, 3.0f, NULL);
+        snprintf(name, sizeof name, "sgemm_%s_qxx (C=999 in)", mname[m]);
+        report(name, packed_finish(&c, 0), tol32(1000), st);
+
+        /* the demos' integer patterns with the launchers' alpha = 5, beta = 3:
+         * exact in FP32 (every partial sum below 2^24) */
+        packed_setup(&c, 0, ta, tb, 96, 80, 700, SEM_SCALAR, 5.0, 3.0, ORACLE_PATTERN11, 0);
+        st = qxx[m](c.M, c.N, c.K, c.dA, c.dB, c.dC, 5.0f, 3.0f, NULL);
+        snprintf(name, sizeof name, "sgemm_%s_qxx integer", mname[m]);
+        report(name, packed_finish(&c, 1), 0.0, st);
+    }
+    /* by-name launches with the reference's argument words */
+    packed_setup(&c, 1, 'T', 'N', 200, 136, 300, SEM_ACC, 1.0, 1.0, ORACLE_UNIFORM, 150);
+    {
+        uint64_t w[11] = {200, 136, 136, 200, 136, 300, 0, 1, (uint64_t)(uintptr_t)c.dA, (uint64_t)(uintptr_t)c.dB,
+                          (uint64_t)(uintptr_t)c.dC};
+        const int st = blasdsp_group_exec("launch_dgemm_TN", 8, 3, w, NULL);
+        report("group_exec launch_dgemm_TN", packed_finish(&c, 0), 1e-13, st);
+    }
+    packed_setup(&c, 0, 'N', 'N', 96, 80, 700, SEM_SCALAR, 5.0, 3.0, ORACLE_PATTERN11, 0);
+    {
+        uint64_t w[7] = {96, 80, 700, 0, (uint64_t)(uintptr_t)c.dA, (uint64_t)(uintptr_t)c.dB,
+                         (uint64_t)(uintptr_t)c.dC};
+        const int st = blasdsp_group_exec("launch_sgemm_NN", 4, 3, w, NULL);
+        report("group_exec launch_sgemm_NN integer", packed_finish(&c, 1), 0.0, st);
+    }
+    packed_setup(&c, 0, 'N', 'N', 3000, 24, 2000, SEM_SCALAR, 5.0, 3.0, ORACLE_UNIFORM, 160);
+    {
+        uint64_t w[8] = {3000, 24, 2000, 0, 1, (uint64_t)(uintptr_t)c.dA, (uint64_t)(uintptr_t)c.dB,
+                         (uint64_t)(uintptr_t)c.dC};
+        const int st = blasdsp_group_exec("launch_irr_thinB_sgemm_NN", 5, 3, w, NULL);
+        report("group_exec launch_irr_thinB_sgemm_NN", packed_finish(&c, 0), tol32(2000), st);
+    }
+    packed_setup(&c, 1, 'T', 'T', 16, 2000, 1024, SEM_ACC, 1.0, 1.0, ORACLE_UNIFORM, 170);
+    {
+        uint64_t w[8] = {16, 2000, 1024, 0, 1, (uint64_t)(uintptr_t)c.dA, (uint64_t)(uintptr_t)c.dB,
+                         (uint64_t)(uintptr_t)c.dC};
+        const int st = blasdsp_group_exec("launch_irr_shortA_dgemm_TT", 5, 3, w, NULL);
+        report("group_exec launch_irr_shortA_dgemm_TT", packed_finish(&c, 0), 1e-13, st);
+    }
+    /* the empty kernels of every family */
+    {
+        int bad = 0;
+        bad |= sgemm_null(8, 8, 8, NULL, NULL, NULL, 1.0f, 1.0f, NULL) != BLASDSP_ERR_INVALID;
+        packed_setup(&c, 1, 'N', 'N', 8, 8, 8, SEM_ACC, 0.0, 1.0, ORACLE_UNIFORM, 180);
+        bad |= irr_shortA_dgemm_null(8, 8, 8, c.dA, c.dB, c.dC, NULL) != 0;
+        bad |= irr_thinB_dgemm_null(8, 8, 8, c.dA, c.dB, c.dC, NULL) != 0;
+        bad |= irr_shortA_sgemm_null(8, 8, 8, c.dA, c.dB, c.dC, NULL) != 0;
+        bad |= sgemm_null(8, 8, 8, c.dA, c.dB, c.dC, 5.0f, 3.0f, NULL) != 0;
+        bad |= irr_thinB_sgemm_null(8, 8, 8, c.dA, c.dB, c.dC, 5.0f, 3.0f, NULL) != 0;
+        uint64_t w[7] = {8, 8, 8, 0, (uint64_t)(uintptr_t)c.dA, (uint64_t)(uintptr_t)c.dB, (uint64_t)(uintptr_t)c.dC};
+        bad |= blasdsp_group_exec("launch_irr_thinB_dgemm_null", 4, 3, w, NULL) != 0;
+        /* C untouched: alpha = 0 makes R the input C */
+        report("*_null (C untouched)", bad ? 1.0 : packed_finish(&c, 1), 0.0, BLASDSP_SUCCESS);
+    }
+}
+
 /* The device generator against the host one (uniform draws are bit-exact). */
 static void check_fill(void)
 {
@@ -288,20 +468,22 @@ int main(void)
     printf("library %s\n", blasdsp_version());
     fflush(stdout);
     const dgemm_fn dfn[4] = {dgemm_NN, dgemm_NT, dgemm_TN, dgemm_TT};
-    const sgemm_fn sfn[4] = {sgemm_NN, sgemm_NT, sgemm_TN, sgemm_TT};
-    const dgemm_fn dshort[4] = {irr_shortA_dgemm_NN, irr_shortA_dgemm_NT, irr_shortA_dgemm_TN, irr_shortA_dgemm_TT};
-    const sgemm_fn sthin[4] = {irr_thinB_sgemm_NN, irr_thinB_sgemm_NT, irr_thinB_sgemm_TN, irr_thinB_sgemm_TT};
+    const sgemm_fn sfn[4] = {blasdsp_sgemm_NN, blasdsp_sgemm_NT, blasdsp_sgemm_TN, blasdsp_sgemm_TT};
+    const dgemm_fn dshort[4] = {blasdsp_irr_shortA_dgemm_NN, blasdsp_irr_shortA_dgemm_NT, blasdsp_irr_shortA_dgemm_TN,
+                                blasdsp_irr_shortA_dgemm_TT};
+    const sgemm_fn sthin[4] = {blasdsp_irr_thinB_sgemm_NN, blasdsp_irr_thinB_sgemm_NT, blasdsp_irr_thinB_sgemm_TN,
+                               blasdsp_irr_thinB_sgemm_TT};
     const char *mname[4] = {"NN", "NT", "TN", "TT"};
     char name[64];
     for (int m = 0; m < 4; ++m) {
         const char ta = mname[m][0], tb = mname[m][1];
         snprintf(name, sizeof name, "dgemm_%s 200x136x300", mname[m]);
         check_d(name, dfn[m], ta, tb, 200, 136, 300, 1.5, -0.5, ORACLE_UNIFORM, 1, 1e-13);
-        snprintf(name, sizeof name, "sgemm_%s 257x129x1000", mname[m]);
+        snprintf(name, sizeof name, "blasdsp_sgemm_%s 257x129x1000", mname[m]);
         check_s(name, sfn[m], FORM_PLAIN, ta, tb, 257, 129, 1000, 1.5f, -0.5f, 11, tol32(1000));
-        snprintf(name, sizeof name, "irr_shortA_dgemm_%s", mname[m]);
+        snprintf(name, sizeof name, "blasdsp_irr_shortA_dgemm_%s", mname[m]);
         check_d(name, dshort[m], ta, tb, 16, 3000, 2048, 1.0, 1.0, ORACLE_UNIFORM, 21, 1e-13);
-        snprintf(name, sizeof name, "irr_thinB_sgemm_%s", mname[m]);
+        snprintf(name, sizeof name, "blasdsp_irr_thinB_sgemm_%s", mname[m]);
         check_s(name, sthin[m], FORM_PLAIN, ta, tb, 3000, 24, 3000, 1.0f, 1.0f, 31, tol32(3000));
     }
     check_d("dgemm_NN integer %11 x %7", dgemm_NN, 'N', 'N', 130, 140, 1500, 1.0, 0.0, ORACLE_PATTERN11, 0, 0.0);
@@ -309,6 +491,7 @@ int main(void)
     /* FP8 vs the quantized oracle: the matrix cores' 32-product group sums
      * carry 13 bits below the group's largest product */
     check_s("fp8 TN 150x170x700", NULL, FORM_FP8, 'T', 'N', 150, 170, 700, 1.0f, 0.5f, 43, ldexp(1.0, -11));
+    check_packed();
     check_grouped();
     check_fill();
     /* check_serve() is switched off: in one run of this program the serving

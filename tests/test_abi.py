@@ -16,7 +16,11 @@ def test_library_exports_every_declared_function(blas):
     assert len(names) >= 30
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
-    for expected in ("dgemm_NN", "dgemm_TT", "sgemm_NT", "irr_shortA_dgemm_TN", "irr_thinB_sgemm_TT",
+    for expected in ("dgemm_NN", "dgemm_TT", "sgemm_NT_qxx", "sgemm_TN_qxx", "sgemm_null",
+                     "irr_shortA_dgemm_TN_dbuf", "irr_shortA_dgemm_null", "irr_shortA_sgemm_TT",
+                     "irr_shortA_sgemm_null", "irr_thinB_dgemm_NN_dbuf", "irr_thinB_dgemm_null",
+                     "irr_thinB_sgemm_TT", "irr_thinB_sgemm_null", "blasdsp_irr_thinB_sgemm_TT",
+                     "blasdsp_sgemm_NT", "blasdsp_group_exec",
                      "dgemm_null", "blasdsp_dgemm", "blasdsp_sgemm_ex", "blasdsp_fill_f32"):
         assert expected in names
 
@@ -76,8 +80,13 @@ def test_null_pointers_rejected(blas):
     assert st == blas.ERR_INVALID
     st = L.dgemm_NN(4, 4, 4, 4, 4, 4, FAKE, None, FAKE, 1.0, 1.0, None)
     assert st == blas.ERR_INVALID
-    st = L.irr_thinB_sgemm_NN(4, 4, 4, 4, 4, 4, FAKE, FAKE, None, 1.0, 1.0, None)
+    st = L.blasdsp_irr_thinB_sgemm_NN(4, 4, 4, 4, 4, 4, FAKE, FAKE, None, 1.0, 1.0, None)
     assert st == blas.ERR_INVALID
+    # the reference's packed forms: checked before anything is written
+    assert L.irr_thinB_sgemm_NN(4, 4, 4, FAKE, FAKE, None, 1.0, 1.0, None) == blas.ERR_INVALID
+    assert L.sgemm_NN_qxx(4, 4, 4, None, FAKE, FAKE, 1.0, 3.0, None) == blas.ERR_INVALID
+    assert L.irr_shortA_dgemm_TT_dbuf(4, 4, 4, FAKE, None, FAKE, None) == blas.ERR_INVALID
+    assert L.irr_shortA_sgemm_NT(-1, 4, 4, FAKE, FAKE, FAKE, None) == blas.ERR_INVALID
 
 
 def test_empty_problems_are_noops(blas):
@@ -85,7 +94,9 @@ def test_empty_problems_are_noops(blas):
     L = blas.lib()
     assert L.blasdsp_dgemm(b"N", b"N", 0, 5, 5, 1.0, None, 5, None, 5, 0.0, None, 5, None) == 0
     assert L.blasdsp_sgemm(b"T", b"T", 5, 0, 5, 1.0, None, 5, None, 5, 0.0, None, 1, None) == 0
-    assert L.sgemm_TT(5, 5, 1, 0, 0, 5, None, None, None, 1.0, 0.0, None) == 0
+    assert L.blasdsp_sgemm_TT(5, 5, 1, 0, 0, 5, None, None, None, 1.0, 0.0, None) == 0
+    assert L.sgemm_TT_qxx(0, 5, 5, None, None, None, 1.0, 3.0, None) == 0
+    assert L.irr_thinB_dgemm_NT_dbuf(5, 0, 5, None, None, None, None) == 0
 
 
 def test_bad_algo_and_fill_arguments(blas):
@@ -179,3 +190,22 @@ def test_stencil_rejects_bad_arguments(blas):
     assert L.stencil_vector(-2, 4, a, b, None) == blas.ERR_INVALID
     assert L.stencil_scalar(4, 4, a, a + 8, None) == blas.ERR_INVALID               # overlaps A
     assert L.stencil_null(-1, 4, a, b, None) == blas.ERR_INVALID
+
+
+def test_group_exec_names_and_argument_counts(blas):
+    """The by-name launcher takes the reference launchers' names and word
+    counts (scalars, then 3 pointers) and rejects anything else on the host."""
+    L = blas.lib()
+    words = (ctypes.c_uint64 * 11)(*([4] * 8 + [0, 0, 0]))  # only name and count checks reach it
+    assert L.blasdsp_group_exec(b"launch_dgemm_XX", 7, 3, words, None) == blas.ERR_UNSUPPORTED
+    assert L.blasdsp_group_exec(b"launch_dgemm_NN", 8, 3, words, None) == blas.ERR_INVALID   # TN has 8
+    assert L.blasdsp_group_exec(b"launch_dgemm_TN", 7, 3, words, None) == blas.ERR_INVALID
+    assert L.blasdsp_group_exec(b"launch_sgemm_NN", 4, 2, words, None) == blas.ERR_INVALID
+    assert L.blasdsp_group_exec(b"launch_irr_thinB_dgemm_NN", 4, 3, words, None) == blas.ERR_INVALID  # 5 words
+    assert L.blasdsp_group_exec(None, 4, 3, words, None) == blas.ERR_INVALID
+    # dims 4 x 4 x 4 with NULL operand pointers: the argument checks refuse them
+    nulls = (ctypes.c_uint64 * 7)(4, 4, 4, 0, 0, 0, 0)
+    assert L.blasdsp_group_exec(b"launch_irr_shortA_sgemm_NN", 4, 3, nulls, None) == blas.ERR_INVALID
+    huge = (ctypes.c_uint64 * 7)(2 ** 63, 4, 4, 0, 1, 1, 1)
+    assert L.blasdsp_group_exec(b"launch_sgemm_NN", 4, 3, huge, None) == blas.ERR_INVALID
+

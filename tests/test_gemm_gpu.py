@@ -207,7 +207,7 @@ def test_reference_entry_points(orc, blas, dev):
                 B = to_dev(stored(orc, K, N, mode[1] == "T", orc.PATTERN7, 0, prec), dev)
                 C = to_dev(orc.matrix(M, N, orc.PATTERN7, 5, dtype=dtype_np(prec), ld=N + 1), dev)
                 C_in = C.cpu().numpy().astype(np.int64)
-                fn = getattr(L, f"{fam}{T}_{mode}")
+                fn = getattr(L, f"dgemm_{mode}" if fam == "" and T == "dgemm" else f"blasdsp_{fam}{T}_{mode}")
                 st = fn(A.stride(0), B.stride(0), C.stride(0), M, N, K, A.data_ptr(), B.data_ptr(),
                         C.data_ptr(), 1.0, 1.0, s)
                 assert st == 0
