@@ -129,6 +129,7 @@ def _declare(L: ctypes.CDLL) -> None:
             getattr(L, name).argtypes = packed + [_vp]
     L.blasdsp_group_exec.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_uint64), _vp]
+    L.blasdsp_gemm_noalloc.argtypes = [ctypes.POINTER(GemmDesc), ctypes.c_int, ctypes.c_int, _vp]
     L.blasdsp_fill_f64.argtypes = [_vp, _i64, _i64, _i64, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _d, _vp]
     L.blasdsp_fill_f32.argtypes = [_vp, _i64, _i64, _i64, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _f, _vp]
     L.blasdsp_sgemm_split_bf16.argtypes = gen + [_f, _vp, _i64, _vp, _i64, _f, _vp, _i64, ctypes.c_int, _vp]

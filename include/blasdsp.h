@@ -328,6 +328,21 @@ BLASDSP_API int blasdsp_group_plan_run(blasdsp_group_plan *plan, const blasdsp_g
                                        int count, void *stream);
 BLASDSP_API int blasdsp_group_plan_destroy(blasdsp_group_plan *plan);
 
+/* ---- allocation-free GEMM --------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): blasdsp_dgemm_ex / blasdsp_sgemm_ex (by desc->dtype)
+ * with the same kernel, tile and K-slice choice, and so the same bits, but
+ * the split-K partial sums go to a device array the library holds statically
+ * (BLASDSP_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends)
+ * instead of a hipMalloc'd workspace.  The call allocates nothing.
+ * A slice count the library picks (splitk <= 0) is lowered until its partials
+ * (slices x M x N elements) fit; an explicit splitk whose partials exceed the
+ * array returns BLASDSP_ERR_NOMEM.  All calls on one GPU share its array, so
+ * calls that overlap on one GPU (two streams or two host threads) race. */
+#define BLASDSP_NOALLOC_SCRATCH_BYTES (64LL << 20)
+BLASDSP_API int blasdsp_gemm_noalloc(const blasdsp_gemm_desc *desc, int algo, int splitk,
+                                     void *stream);
 
 /* ---- continuous-batching request loop (no counterpart in the reference) ----
  * blasdsp_serve() plays a list of GEMM requests against the GPU as a server

@@ -20,7 +20,7 @@ def test_library_exports_every_declared_function(blas):
                      "irr_shortA_dgemm_TN_dbuf", "irr_shortA_dgemm_null", "irr_shortA_sgemm_TT",
                      "irr_shortA_sgemm_null", "irr_thinB_dgemm_NN_dbuf", "irr_thinB_dgemm_null",
                      "irr_thinB_sgemm_TT", "irr_thinB_sgemm_null", "blasdsp_irr_thinB_sgemm_TT",
-                     "blasdsp_sgemm_NT", "blasdsp_group_exec",
+                     "blasdsp_sgemm_NT", "blasdsp_group_exec", "blasdsp_gemm_noalloc",
                      "dgemm_null", "blasdsp_dgemm", "blasdsp_sgemm_ex", "blasdsp_fill_f32"):
         assert expected in names
 
@@ -209,3 +209,17 @@ def test_group_exec_names_and_argument_counts(blas):
     huge = (ctypes.c_uint64 * 7)(2 ** 63, 4, 4, 0, 1, 1, 1)
     assert L.blasdsp_group_exec(b"launch_sgemm_NN", 4, 3, huge, None) == blas.ERR_INVALID
 
+
+def test_noalloc_rejects_oversized_explicit_split(blas):
+    """An explicit split-K whose partials exceed the static scratch array is
+    refused before any device work (64 MiB: 16 slices of 2048 x 2048 FP64
+    need 512 MiB)."""
+    L = blas.lib()
+    n = 2048
+    d = blas.GemmDesc(blas.FLOAT64, b"N", b"N", n, n, n, 1.0, 0.0, FAKE, n, FAKE, n, FAKE, n)
+    assert L.blasdsp_gemm_noalloc(ctypes.byref(d), blas.ALGO_TILE, 16, None) == blas.ERR_NOMEM
+    assert L.blasdsp_gemm_noalloc(None, blas.ALGO_TILE, 1, None) == blas.ERR_INVALID
+    bad = blas.GemmDesc(7, b"N", b"N", n, n, n, 1.0, 0.0, FAKE, n, FAKE, n, FAKE, n)
+    assert L.blasdsp_gemm_noalloc(ctypes.byref(bad), blas.ALGO_TILE, 1, None) == blas.ERR_INVALID
+    short = blas.GemmDesc(blas.FLOAT32, b"N", b"N", n, n, n, 1.0, 0.0, FAKE, n - 1, FAKE, n, FAKE, n)
+    assert L.blasdsp_gemm_noalloc(ctypes.byref(short), blas.ALGO_AUTO, 0, None) == blas.ERR_INVALID
