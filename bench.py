@@ -72,6 +72,35 @@ def traffic_from_profiles(name="dgemm_4096_hbm_traffic.json"):
         return None
 
 
+def cpu_threads():
+    """CPUs this process may run on (its affinity mask, capped by
+    OMP_NUM_THREADS when that is set): all the host cores the run is given."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def _cpu_timed(run, work_of, probe, limit, target_s, step=1):
+    """Calibrate a sample size on `probe` units so that one run takes about
+    target_s, then the median of 3 runs after that warm-up (units rounded
+    down to a multiple of `step`, at most `limit`)."""
+    t = run(probe)
+    units = int(min(limit, max(probe, probe * target_s / max(t, 1e-3))))
+    units = max(step, units // step * step)
+    times = [run(units) for _ in range(3)]
+    return units, statistics.median(times), work_of(units)
+
+
+def _cpu_record(value, unit, threads, sample):
+    return {"value": value, "unit": unit, "cores": threads, "kind": "port",
+            "host_cpus": os.cpu_count(), "sample": sample}
+
+
 def cpu_baseline(target_s: float = 4.0):
     """The oracle's reference-tree dgemm (dgemm_demo.c:233-252 with the BLAS
     epilogue) on R rows x 4096 columns x K = 4096, median of 3 after a warm-up;
@@ -80,7 +109,7 @@ def cpu_baseline(target_s: float = 4.0):
     import numpy as np
     import oracle as orc
 
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     orc.set_threads(threads)
     A = orc.matrix(N_DIM, N_DIM, orc.UNIFORM, 1, ld=LD)
     B = orc.matrix(N_DIM, N_DIM, orc.UNIFORM, 2, ld=LD)
@@ -92,22 +121,70 @@ def cpu_baseline(target_s: float = 4.0):
         orc.gemm_blas_host(A[:rows], B, C, ALPHA, BETA)
         return time.perf_counter() - t0
 
-    probe = max(threads, 8)
-    t = run(probe)  # warm-up and calibration
-    rows = int(min(N_DIM, max(probe, probe * target_s / max(t, 1e-3))))
-    rows = max(threads, rows // threads * threads)
-    times = [run(rows) for _ in range(3)]
-    tmed = statistics.median(times)
-    flops = 2.0 * rows * N_DIM * N_DIM
-    return {
-        "value": round(flops / tmed / 1e12, 6),
-        "unit": "TFLOP/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": (f"oracle dgemm (reference K_G=512 fma tree, BLAS epilogue), NN, {rows} rows x 4096 x "
-                   f"K=4096 of the same workload, gcc -O3 -fopenmp, median of 3 runs of "
-                   f"{tmed:.2f} s"),
-    }
+    rows, tmed, flops = _cpu_timed(run, lambda r: 2.0 * r * N_DIM * N_DIM, max(threads, 8), N_DIM, target_s,
+                                   step=threads)
+    return _cpu_record(round(flops / tmed / 1e12, 6), "TFLOP/s", threads,
+                       f"oracle dgemm (reference K_G=512 fma tree, BLAS epilogue), NN, {rows} rows x 4096 x "
+                       f"K=4096 of the same workload, gcc -O3 -fopenmp, {threads} threads, median of 3 runs of "
+                       f"{tmed:.2f} s")
+
+
+def cpu_baseline_skinny(target_s: float = 3.0):
+    """The restated host loop of irr_shortA_dgemm_demo.c:205-224 (the dgemm
+    K_G = 512 fma tree, BLAS epilogue) on the shortA FP64 case M = 32,
+    K = 8192, over a sample of the 65536 columns; median of 3 after a warm-up.
+    Reported in TFLOP/s and in GB/s of the case's algorithmic bytes."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import numpy as np
+    import oracle as orc
+
+    threads = cpu_threads()
+    orc.set_threads(threads)
+    M, K, LONG = 32, 8192, 65536
+    A = orc.matrix(M, K, orc.NORMAL, 3)
+
+    def run(cols):
+        B = orc.matrix(K, cols, orc.NORMAL, 2)
+        C = orc.matrix(M, cols, orc.NORMAL, 4)
+        t0 = time.perf_counter()
+        orc.gemm_blas_host(A, B, C, 1.0, 1.0)
+        return time.perf_counter() - t0
+
+    cols, tmed, flops = _cpu_timed(run, lambda c: 2.0 * M * c * K, 256, LONG, target_s, step=64)
+    nbytes = 8.0 * (M * K + K * cols + 2 * M * cols)
+    rec = _cpu_record(round(flops / tmed / 1e12, 6), "TFLOP/s", threads,
+                      f"oracle reference-tree dgemm (irr_shortA_dgemm_demo.c:205-224, BLAS epilogue), shortA FP64 "
+                      f"M=32 K=8192, {cols} of the 65536 columns, gcc -O3 -fopenmp, {threads} threads, median of "
+                      f"3 runs of {tmed:.2f} s")
+    rec["gbs"] = round(nbytes / tmed / 1e9, 3)
+    return rec
+
+
+def cpu_baseline_sgemm(n: int = 8192, target_s: float = 3.0):
+    """The restated host loop of sgemm_demo.c:214-242 (K_G = 512 fma chains in
+    double, BLAS epilogue) on R rows of the SGEMM NN n^3 problem, U[-1,1);
+    median of 3 after a warm-up."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import numpy as np
+    import oracle as orc
+
+    threads = cpu_threads()
+    orc.set_threads(threads)
+    B = orc.matrix(n, n, orc.UNIFORM, 4, dtype=np.float32)
+    Afull = orc.matrix(n, n, orc.UNIFORM, 3, dtype=np.float32)
+
+    def run(rows):
+        A = Afull[:rows]
+        C = np.zeros((rows, n), dtype=np.float32)
+        t0 = time.perf_counter()
+        orc.gemm_blas_host(A, B, C, 1.0, 0.0)
+        return time.perf_counter() - t0
+
+    rows, tmed, flops = _cpu_timed(run, lambda r: 2.0 * r * n * n, max(threads, 8), n, target_s, step=threads)
+    return _cpu_record(round(flops / tmed / 1e12, 6), "TFLOP/s", threads,
+                       f"oracle reference-tree sgemm (sgemm_demo.c:214-242, BLAS epilogue), NN, {rows} rows x "
+                       f"{n} x K={n} of the same workload, gcc -O3 -fopenmp, {threads} threads, median of 3 runs "
+                       f"of {tmed:.2f} s")
 
 
 def main():
@@ -293,7 +370,10 @@ def bench_skinny(args):
                   {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                    "frac": round(gbs / PEAK_HBM_GBS, 4),
                    "traffic": traffic_from_profiles("skinny_k8192_hbm_traffic.json"),
-                   "algorithmic_bytes_per_launch": round(tot_bytes / len(cases))},
+                   "algorithmic_bytes_per_launch": round(tot_bytes / len(cases)),
+                   "min_case_gbs": min(c["gbs"] for c in cases),
+                   "cases_at_or_above_6400_gbs": sum(c["gbs"] >= 6400.0 for c in cases)},
+                  None if args.no_cpu else cpu_baseline_skinny(),
                   ms_per_step=round(tot_ms / len(cases), 4))
     print(json.dumps(out), flush=True)
 
@@ -344,7 +424,8 @@ def bench_precision(args):
                    "split_bf16_6_over_fp32": round(u["fp32_mfma"]["ms"] / u["split_bf16_6"]["ms"], 3),
                    "fp8_over_fp32": round(u["fp32_mfma"]["ms"] / u["fp8_e4m3_block128"]["ms"], 3)},
                   {"bound": "mfma", "achieved": round(u[best]["tflops"] * 3, 2), "peak": 2500.0,
-                   "unit": "TFLOP/s", "frac": round(u[best]["tflops"] * 3 / 2500.0, 4), "traffic": None})
+                   "unit": "TFLOP/s", "frac": round(u[best]["tflops"] * 3 / 2500.0, 4), "traffic": None},
+                  None if args.no_cpu else cpu_baseline_sgemm())
     print(json.dumps(out), flush=True)
 
 
@@ -520,7 +601,7 @@ def bench_sgemm512(args):
     this size), and by the C demo with the reference harness's method (calls
     timed from C, the empty kernel subtracted).  CPU: the oracle's restatement of the
     reference host GEMM (sgemm_demo.c:214-242, BLAS epilogue) on the whole
-    problem, min(16, cores) threads, median of 5; the two results are compared."""
+    problem, on every CPU the run is given, median of 5; the two results are compared."""
     import numpy as np
     import torch
     import __graft_entry__
@@ -561,7 +642,7 @@ def bench_sgemm512(args):
         demo["returncode"] = r.returncode
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
         orc.set_threads(threads)
         times = []
         for _ in range(6):
@@ -571,10 +652,10 @@ def bench_sgemm512(args):
             times.append(time.perf_counter() - t0)
         tmed = statistics.median(times[1:])
         cpu_err = orc.normwise(C, ref)[0]
-        cpu = {"value": round(flops / tmed / 1e9, 3), "unit": "GFLOP/s", "cores": threads, "kind": "port",
-               "sample": f"the whole SGEMM NN 512^3 through the oracle's reference-tree sgemm (BLAS epilogue), "
-                         f"median of 5 runs of {tmed * 1e3:.2f} ms; normwise error vs the extended-precision "
-                         f"oracle {cpu_err:.2e}"}
+        cpu = _cpu_record(round(flops / tmed / 1e9, 3), "GFLOP/s", threads,
+                          f"the whole SGEMM NN 512^3 through the oracle's reference-tree sgemm (BLAS epilogue), "
+                          f"{threads} threads, median of 5 runs of {tmed * 1e3:.2f} ms; normwise error vs the "
+                          f"extended-precision oracle {cpu_err:.2e}")
     out = _common(args, round(flops / ms / 1e6, 3), "GFLOP/s", "f32",
                   {"workload": "SGEMM NN 512^3, lda=K, ldb=ldc=N, alpha=1, beta=1, U[-1,1) seed 0",
                    "us_per_call": round(ms * 1e3, 2), "normwise_err_vs_oracle": err,
@@ -584,11 +665,42 @@ def bench_sgemm512(args):
     print(json.dumps(out), flush=True)
 
 
+# The reference's shape sweeps, (M, K, N) in the order of its scripts
+# (dgemm_demo / irr_thinB_sgemm_demo take dev_id M K N): every distinct shape
+# of dgemm_release_V1.1/bin/run.sh:2-140 (Fig. 1 (a)-(c), the Fig. 2 weak-scaling
+# shapes, the odd-N edge cases) and irr_thinB_sgemm_release_V0.0/bin/run.sh:15-233.
+SWEEP_DGEMM = [
+    (1512, 2048, 1536), (3024, 2048, 1536), (6048, 2048, 1536), (12096, 2048, 1536), (24192, 2048, 1536),
+    (48384, 2048, 1536), (3024, 512, 1536), (3024, 1024, 1536), (3024, 4096, 1536), (3024, 8192, 1536),
+    (3024, 16384, 1536), (3024, 20480, 48), (3024, 20480, 96), (3024, 20480, 192), (3024, 2048, 384),
+    (3024, 2048, 768), (3024, 2048, 3072), (3024, 2048, 6144), (1512, 2048, 3072), (1512, 2048, 6144),
+    (1512, 2048, 12288), (3024, 2048, 12288), (6048, 2048, 3072), (6048, 2048, 6144), (6048, 2048, 12288),
+    (12096, 2048, 3072), (12096, 2048, 6144), (12096, 2048, 12288), (3024, 512, 3072), (3024, 512, 6144),
+    (3024, 512, 12288), (3024, 1024, 3072), (3024, 1024, 6144), (3024, 1024, 12288), (3024, 4096, 3072),
+    (3024, 4096, 6144), (3024, 4096, 12288), (3024, 20480, 384), (3024, 20480, 768), (3024, 20480, 1536),
+    (96768, 2048, 1536), (6048, 128, 1536), (6048, 256, 1536), (6048, 512, 1536), (6048, 1024, 1536),
+    (6048, 4096, 1536), (6048, 8192, 1536), (6048, 16384, 1536), (6048, 20480, 48), (6048, 20480, 96),
+    (6048, 20480, 192), (6048, 2048, 384), (6048, 2048, 768), (6048, 2048, 24576), (2304, 2048, 1536),
+    (579, 1022, 63), (579, 1022, 64), (579, 1022, 65), (579, 1022, 128), (579, 1022, 129), (579, 1022, 191),
+    (579, 1022, 192), (579, 1022, 193), (579, 1022, 255), (579, 1022, 256), (579, 1022, 257), (579, 1022, 319),
+    (579, 1022, 320), (579, 1022, 321), (579, 1022, 383), (579, 1022, 384), (579, 1022, 385), (579, 1022, 447),
+    (579, 1022, 448), (579, 1022, 449), (579, 1022, 512), (579, 1022, 513), (579, 1022, 576), (579, 1022, 577),
+    (1311, 1112, 128), (2304, 2048, 3072), (2304, 2048, 6144), (2304, 2048, 12288), (2304, 2048, 18432),
+    (1152, 2048, 1536)]
+SWEEP_THINB_S = [
+    (24576, 4096, 192), (24576, 6144, 200), (512, 2048, 300), (700, 24576, 200), (700, 33278, 300),
+    (65536, 16, 16), (65536, 32, 32), (65536, 48, 48), (65536, 64, 64), (65536, 80, 80), (65536, 96, 96),
+    (20480, 20480, 16), (20480, 20480, 32), (20480, 20480, 48), (20480, 20480, 64), (20480, 20480, 80),
+    (20480, 20480, 96), (131072, 32, 32), (262144, 32, 32), (524288, 32, 32), (1048576, 32, 32),
+    (2097152, 32, 32), (4194304, 32, 32), (8192, 8192, 32), (10240, 10240, 32), (12288, 12288, 32),
+    (14336, 14336, 32), (16384, 16384, 32), (18432, 18432, 32)]
+
+
 def bench_sweep(args):
-    """The reference's published shape sweeps on this GPU: dgemm Fig. 1 (a)-(c)
-    (dgemm_release_V1.1/bin/run.sh: M, K, N varied around 3024 x 2048 x 1536,
-    all four modes) and irr_thinB_sgemm (irr_thinB_sgemm_release_V0.0/bin/run.sh:
-    M = 65536 with N = K in 16..96, and M = K = 20480 with N in 16..96)."""
+    """The reference's published shape sweeps on this GPU, one JSON line per
+    shape (every mode), then the summary line: dgemm through dgemm_NN..TT
+    (FP64, alpha = beta = 1) and irr_thinB_sgemm through its leading-dimension
+    form blasdsp_irr_thinB_sgemm_* (FP32, alpha = beta = 1), U[-1,1)."""
     import torch
     import __graft_entry__
 
@@ -596,61 +708,47 @@ def bench_sweep(args):
     stream = torch.cuda.current_stream()
     L = blas.lib()
     reps = max(2, min(args.steps, 5))
-    dg = []
-    shapes = ([(m, 2048, 1536) for m in (1512, 3024, 6048, 12096, 24192, 48384)] +
-              [(3024, k, 1536) for k in (512, 1024, 4096, 8192, 16384)] +
-              [(3024, 20480, n) for n in (48, 96, 192)] + [(3024, 2048, n) for n in (384, 768, 3072, 6144)])
+
+    def run_shape(dt, M, K, N, fn_of, s):
+        A = torch.empty((M, K), dtype=dt, device="cuda")
+        AT = torch.empty((K, M), dtype=dt, device="cuda")
+        B = torch.empty((K, N), dtype=dt, device="cuda")
+        BT = torch.empty((N, K), dtype=dt, device="cuda")
+        C = torch.empty((M, N), dtype=dt, device="cuda")
+        for X, sd in ((A, 1), (AT, 2), (B, 3), (BT, 4), (C, 5)):
+            blas.fill(X, X.shape[0], X.shape[1], blas.UNIFORM, sd)
+        ms_by = {}
+        for mode in MODES:
+            a = A if mode[0] == "N" else AT
+            b = B if mode[1] == "N" else BT
+            fn = fn_of(mode)
+            lda, ldb = a.shape[1], b.shape[1]
+            ms_by[mode] = _time_events(lambda: blas.check(mode, fn(lda, ldb, N, M, N, K, a.data_ptr(), b.data_ptr(),
+                                                                    C.data_ptr(), 1.0, 1.0, stream.cuda_stream)),
+                                       reps, stream)
+        del A, AT, B, BT, C
+        flops, nbytes = 2.0 * M * N * K, s * (M * K + K * N + 2.0 * M * N)
+        return {m: {"ms": round(t, 4), "tflops": round(flops / t / 1e9, 2), "gbs": round(nbytes / t / 1e6, 1)}
+                for m, t in ms_by.items()}, flops * 4, sum(ms_by.values())
+
     tot_f = tot_ms = 0.0
-    for M, K, N in shapes:
-        A = torch.empty((M, K), dtype=torch.float64, device="cuda")
-        AT = torch.empty((K, M), dtype=torch.float64, device="cuda")
-        B = torch.empty((K, N), dtype=torch.float64, device="cuda")
-        BT = torch.empty((N, K), dtype=torch.float64, device="cuda")
-        C = torch.empty((M, N), dtype=torch.float64, device="cuda")
-        for X, sd in ((A, 1), (AT, 2), (B, 3), (BT, 4), (C, 5)):
-            blas.fill(X, X.shape[0], X.shape[1], blas.UNIFORM, sd)
-        row = {"M": M, "K": K, "N": N}
-        for mode in MODES:
-            a = A if mode[0] == "N" else AT
-            b = B if mode[1] == "N" else BT
-            fn = getattr(L, "dgemm_" + mode)
-            lda, ldb = a.shape[1], b.shape[1]
-            ms = _time_events(lambda: blas.check(mode, fn(lda, ldb, N, M, N, K, a.data_ptr(), b.data_ptr(),
-                                                           C.data_ptr(), 1.0, 1.0, stream.cuda_stream)),
-                              reps, stream)
-            f = 2.0 * M * N * K
-            row[mode] = round(f / ms / 1e9, 2)
-            tot_f += f
-            tot_ms += ms
-        dg.append(row)
-        del A, AT, B, BT, C
-    tb = []
-    thin = [(65536, k, k) for k in (16, 32, 48, 64, 80, 96)] + [(20480, 20480, n) for n in (16, 32, 48, 64, 80, 96)]
-    for M, K, N in thin:
-        A = torch.empty((M, K), dtype=torch.float32, device="cuda")
-        AT = torch.empty((K, M), dtype=torch.float32, device="cuda")
-        B = torch.empty((K, N), dtype=torch.float32, device="cuda")
-        BT = torch.empty((N, K), dtype=torch.float32, device="cuda")
-        C = torch.empty((M, N), dtype=torch.float32, device="cuda")
-        for X, sd in ((A, 1), (AT, 2), (B, 3), (BT, 4), (C, 5)):
-            blas.fill(X, X.shape[0], X.shape[1], blas.UNIFORM, sd)
-        row = {"M": M, "K": K, "N": N}
-        nbytes = 4.0 * (M * K + K * N + 2 * M * N)
-        for mode in MODES:
-            a = A if mode[0] == "N" else AT
-            b = B if mode[1] == "N" else BT
-            fn = getattr(L, "blasdsp_irr_thinB_sgemm_" + mode)
-            lda, ldb = a.shape[1], b.shape[1]
-            ms = _time_events(lambda: blas.check(mode, fn(lda, ldb, N, M, N, K, a.data_ptr(), b.data_ptr(),
-                                                           C.data_ptr(), 1.0, 1.0, stream.cuda_stream)),
-                              reps, stream)
-            row[mode] = {"tflops": round(2.0 * M * N * K / ms / 1e9, 2), "gbs": round(nbytes / ms / 1e6, 1)}
-        tb.append(row)
-        del A, AT, B, BT, C
+    for M, K, N in SWEEP_DGEMM:
+        res, f, ms = run_shape(torch.float64, M, K, N, lambda m: getattr(L, "dgemm_" + m), 8)
+        tot_f += f
+        tot_ms += ms
+        best = max(v["tflops"] for v in res.values())
+        print(json.dumps({"sweep": "dgemm", "M": M, "K": K, "N": N, "modes": res,
+                          "best_frac_fp64_peak": round(best / 78.6, 4)}), flush=True)
+    for M, K, N in SWEEP_THINB_S:
+        res, _, _ = run_shape(torch.float32, M, K, N, lambda m: getattr(L, "blasdsp_irr_thinB_sgemm_" + m), 4)
+        best = max(v["gbs"] for v in res.values())
+        print(json.dumps({"sweep": "irr_thinB_sgemm", "M": M, "K": K, "N": N, "modes": res,
+                          "best_frac_hbm": round(best / PEAK_HBM_GBS, 4)}), flush=True)
     tf = tot_f / tot_ms / 1e9
     out = _common(args, round(tf, 3), "TFLOP/s", "f64 (dgemm) + f32 (thinB)",
-                  {"workload": "reference shape sweeps: dgemm Fig. 1 (a)-(c), irr_thinB_sgemm N = K in 16..96",
-                   "dgemm_tflops_by_mode": dg, "irr_thinB_sgemm": tb},
+                  {"workload": f"reference shape sweeps: {len(SWEEP_DGEMM)} dgemm shapes x 4 modes "
+                               f"(dgemm_release_V1.1/bin/run.sh), {len(SWEEP_THINB_S)} irr_thinB_sgemm shapes x 4 "
+                               f"modes (irr_thinB_sgemm_release_V0.0/bin/run.sh); value: dgemm aggregate"},
                   {"bound": "mfma", "achieved": round(tf, 3), "peak": 78.6, "unit": "TFLOP/s",
                    "frac": round(tf / 78.6, 4), "traffic": None})
     print(json.dumps(out), flush=True)
@@ -690,7 +788,7 @@ def bench_stencil(args):
     cpu = None
     if not args.no_cpu:
         orc = __graft_entry__.load_oracle()
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
         orc.set_threads(threads)
         n = 8192
         Ah = np.random.default_rng(9).random((n + 2, n + 2))
@@ -704,11 +802,10 @@ def bench_stencil(args):
                 orc.lib().oracle_stencil(n, n, Ah.ctypes.data, n + 2, Bh.ctypes.data, n, 0)
             times.append((time.perf_counter() - t0) / sweeps)
         tmed = statistics.median(times)
-        cpu = {"value": round(n * n / (tmed * 1e3), 1), "unit": "grid points/ms", "cores": threads,
-               "kind": "port",
-               "sample": f"the oracle's restatement of reference_stencil (matrix_stencil.c:33-47) on the whole "
-                         f"8192^2 grid, gcc -O3 -fopenmp, median of 3 runs of {sweeps} sweeps "
-                         f"({tmed * 1e3:.1f} ms per sweep)"}
+        cpu = _cpu_record(round(n * n / (tmed * 1e3), 1), "grid points/ms", threads,
+                          f"the oracle's restatement of reference_stencil (matrix_stencil.c:33-47) on the whole "
+                          f"8192^2 grid, gcc -O3 -fopenmp, {threads} threads, median of 3 runs of {sweeps} sweeps "
+                          f"({tmed * 1e3:.1f} ms per sweep)")
     out = _common(args, big["grids_per_ms"], "grid points/ms", "f64",
                   {"workload": "2-D five-point stencil sweep, FP64, haloed (M+2) x (N+2) grids, M = N in "
                                "{2048, 4096, 6144, 8192} (value: 8192^2)", "cases": cases},
