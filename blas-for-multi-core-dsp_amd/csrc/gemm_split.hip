@@ -17,6 +17,9 @@
 // BF16 forms accumulate in two levels: each 256-deep K chunk from zero in its
 // own MFMA accumulators, then added into the running sum.
 //
+// The six-product BF16 form runs on fragment-tiled pieces in
+// gemm_split_frag.hip; the three-product forms run here.
+//
 // Kernels:
 //   rowscale_kernel  (FP16 form) per-row power-of-two scales of op(A), op(B)^T;
 //   split_kernel     reads op(A) (or op(B)^T) in any storage order and writes
@@ -233,14 +236,6 @@ __global__ void __launch_bounds__(256) split_kernel(const float *__restrict__ X,
 enum { kBF16x6 = 0, kBF16x3 = 1, kFP16x3 = 2 };
 template <int S>
 struct Scheme;
-template <>
-struct Scheme<kBF16x6> {
-    static constexpr int NP = 3, NT = 6, NACC = 1, WPE = 1;
-    static constexpr bool FP16 = false, CHUNK = true;
-    static constexpr int a[6] = {1, 0, 2, 0, 1, 0};
-    static constexpr int b[6] = {1, 2, 0, 1, 0, 0};
-    static constexpr int acc[6] = {0, 0, 0, 0, 0, 0};
-};
 template <>
 struct Scheme<kBF16x3> {
     static constexpr int NP = 2, NT = 3, NACC = 1, WPE = 2;
@@ -510,7 +505,9 @@ size_t split_workspace_bytes(int64_t M, int64_t N, int64_t K, int scheme)
 {
     const int np = scheme == kBF16x6 ? 3 : 2;
     const bool scaled = scheme == kFP16x3;
-    const int64_t Mp = round_up(M, kRowPad), Np = round_up(N, kRowPad), Kp = round_up(K, kKpad);
+    // the six-product form's kernel takes op(A) in 256-row tiles
+    const int64_t Mp = round_up(M, scheme == kBF16x6 ? 2 * kRowPad : kRowPad), Np = round_up(N, kRowPad),
+                  Kp = round_up(K, kKpad);
     return (size_t)np * (size_t)(Mp + Np) * (size_t)Kp * sizeof(uint16_t) +
            (scaled ? ((size_t)2 * (Mp + Np) + (size_t)kScaleSlices * (Mp > Np ? Mp : Np)) * sizeof(float) : 0);
 }
@@ -518,7 +515,7 @@ size_t split_workspace_bytes(int64_t M, int64_t N, int64_t K, int scheme)
 hipError_t launch_split_gemm(const Gemm<float> &g, int scheme, void *work, hipStream_t s)
 {
     switch (scheme) {
-    case kBF16x6: return run_scheme<kBF16x6>(g, work, s);
+    case kBF16x6: return launch_split6_frag(g, work, s);
     case kBF16x3: return run_scheme<kBF16x3>(g, work, s);
     case kFP16x3: return run_scheme<kFP16x3>(g, work, s);
     default: return hipErrorInvalidValue;

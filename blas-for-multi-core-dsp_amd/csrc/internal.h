@@ -60,6 +60,9 @@ bool skinny_fills(const Gemm<T> &g, int num_cus);
 // hi/lo with 3 products.  `work` must hold split_workspace_bytes(...) bytes.
 size_t split_workspace_bytes(int64_t M, int64_t N, int64_t K, int scheme);
 hipError_t launch_split_gemm(const Gemm<float> &g, int scheme, void *work, hipStream_t s);
+// The six-product BF16 form on fragment-tiled planes (gemm_split_frag.hip);
+// `work` must hold split_workspace_bytes(M, N, K, 0) bytes.
+hipError_t launch_split6_frag(const Gemm<float> &g, void *work, hipStream_t s);
 
 // FP32 GEMM on the FP8 (E4M3) matrix cores with per-128-block power-of-two
 // scales (gemm_fp8.hip).  `work` must hold fp8_workspace_bytes(...) bytes.
