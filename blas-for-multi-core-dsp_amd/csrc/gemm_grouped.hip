@@ -1,25 +1,33 @@
 // gemm_grouped.hip — many GEMMs of uneven shapes and transpose modes in one
-// persistent launch per precision.
+// replay of four persistent kernels.
 //
 // No counterpart in the reference (which runs one GEMM per hthread_group_exec,
-// dgemm_release_V1.1/host_code/dgemm_demo.c:151).  The host flattens a group
-// of problems into a list of C tiles: problem p owns tiles
-// [tile_begin(p), tile_begin(p) + tiles_m * tiles_n).  A fixed grid of
-// workgroups (the tile kernel occupancy times the CU count) strides over it;
-// each workgroup finds its problem by binary search over tile_begin and runs
-// the same tile body as the per-call kernel (tile_body.h), so a problem's C is
-// bit-identical alone and in a group.  Problems are sorted by K (the cost of
-// one of their tiles) in decreasing order, so the strided assignment deals the
-// expensive tiles out first.  Each problem picks a 64 x 64 or 128 x 128 tile;
-// there is one kernel per (precision, tile edge), each with the register
-// budget and LDS of its tile size (FP64 128 tiles also one kernel per
-// transpose mode), and the kernels are independent graph nodes that run side
-// by side on the device.
+// dgemm_release_V1.1/host_code/dgemm_demo.c:151).  The host sorts the group's
+// problems into four lists by (precision, tile edge): FP32 and FP64, 128 x 128
+// or 64 x 64 tiles, each list holding every transpose mode.  It flattens each
+// list into C tiles: problem p owns tiles [tile_begin(p), tile_begin(p) +
+// tiles_m * tiles_n), problems sorted by K (the cost of one of their tiles) in
+// decreasing order.  Each list has its own kernel with the register budget and
+// LDS of its tile size; a fixed grid of workgroups (occupancy times CU count)
+// takes tiles from the list's atomic counter one at a time, so the expensive
+// tiles go first and a workgroup that starts late (its CU slot held by
+// another list's kernel) takes whatever is left.  A workgroup finds its
+// problem by binary search over tile_begin and runs the same tile body as the
+// per-call kernel (tile_body.h), so a problem's C is bit-identical alone and
+// in a group.  The four kernels are independent graph nodes; a process has
+// four hardware queues, so they start together and share the device.
 //
-// The problem list lives in device memory and the grid does not depend on it,
+// The problem list lives in device memory and the grids do not depend on it,
 // so one instantiated hipGraph (built from explicit kernel nodes, no stream
-// capture and no memset nodes) replays any group: the host uploads the list
-// and launches the graph.
+// capture and no memset nodes) replays any group: the host uploads the list,
+// with the tile counters at zero, and launches the graph.
+//
+// On 512 requests of a mixed workload (M, N, K log-uniform in [16, 4096], 30 %
+// with one side 8-32, 70 % FP32, every mode, unaligned leading dimensions) on
+// MI355X: seven kernels (FP64 128 tiles one per mode) with a fixed stride
+// over the tiles took 5.8 ms, the three FP64 128 kernels starting only when
+// one of the first four had finished; four kernels with the tile counter 5.4
+// ms; loads at any alignment in the tile body 4.5 ms.
 #include <stdlib.h>
 
 #include <algorithm>
@@ -36,9 +44,9 @@ using tile::Cfg;
 
 // Waves per workgroup: 128 tiles take the 8-wave body (32 x 64 per wave) of
 // the per-call kernel, 64 tiles 4 waves (32 x 32 each).  With the group's
-// problem lookup the 8-wave FP64 body spills 4-25 registers; on 512 requests
-// of a mixed workload (M, N, K log-uniform in [16, 4096], FP32 and FP64) it
-// still ran 3 % faster than 4 waves for FP64, and 8 waves for both
+// problem lookup and all four modes the 8-wave FP64 body spills 16
+// registers; on the mixed workload above, with one kernel per mode (4-25
+// spilled), it ran 3 % faster than 4 waves for FP64, and 8 waves for both
 // precisions 7 % faster than 4 for both.
 template <typename T, int BM>
 constexpr int group_nw()
@@ -57,21 +65,29 @@ __device__ __forceinline__ void run_tile(const GroupProblem &P, int64_t local, T
     g.B = (const T *)P.B; g.ldb = P.ldb;
     g.C = (T *)P.C; g.ldc = P.ldc;
     const int64_t tm = local / P.tiles_n, tn = local % P.tiles_n;
-    tile::tile_body<T, BM, TA, TB, group_nw<T, BM>()>(g, tm * BM, tn * BM, P.a_vec, P.b_vec, lds);
+    tile::tile_body<T, BM, TA, TB, group_nw<T, BM>()>(g, tm * BM, tn * BM, lds);
 }
 
-// MODE < 0: the problem's transpose mode is read per tile; else fixed.
-template <typename T, int BM, int MODE>
+template <typename T, int BM>
 __global__ void __launch_bounds__((64 * group_nw<T, BM>()))
 __attribute__((amdgpu_waves_per_eu(Cfg<T, BM, group_nw<T, BM>()>::WPE, Cfg<T, BM, group_nw<T, BM>()>::WPE)))
-grouped_kernel(const GroupHeader *__restrict__ hdr, const GroupProblem *__restrict__ probs)
+grouped_kernel(GroupHeader *__restrict__ hdr, const GroupProblem *__restrict__ probs)
 {
+    // The tile number is passed to the waves through the first 8 bytes of the
+    // stage buffers (a separate LDS variable would cost the 64-tile kernels a
+    // workgroup per CU), so it is read by every wave before the body writes.
     __shared__ __attribute__((aligned(16))) T lds[Cfg<T, BM, group_nw<T, BM>()>::LDS_ELEMS];
-    const int w = group_list(sizeof(T) == 8, BM, MODE < 0 ? 0 : MODE);
+    unsigned long long *const s_tile = reinterpret_cast<unsigned long long *>(lds);
+    const int w = group_list(sizeof(T) == 8, BM);
     const int count = hdr->count[w];
     const int64_t total = hdr->total_tiles[w];
     probs += hdr->first[w];
-    for (int64_t t = blockIdx.x; t < total; t += gridDim.x) {
+    for (;;) {
+        if (threadIdx.x == 0) *s_tile = atomicAdd(&hdr->next[w], 1ull);
+        __syncthreads();
+        const int64_t t = (int64_t)*s_tile;
+        __syncthreads();
+        if (t >= total) break;  // the same value in every wave: all leave together
         int lo = 0, hi = count - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -80,13 +96,13 @@ grouped_kernel(const GroupHeader *__restrict__ hdr, const GroupProblem *__restri
         }
         const GroupProblem &P = probs[lo];
         const int64_t local = t - P.tile_begin;
-        switch (MODE < 0 ? ((P.ta << 1) | P.tb) : MODE) {
+        switch ((P.ta << 1) | P.tb) {
         case 0: run_tile<T, BM, false, false>(P, local, lds); break;
         case 1: run_tile<T, BM, false, true>(P, local, lds); break;
         case 2: run_tile<T, BM, true, false>(P, local, lds); break;
         default: run_tile<T, BM, true, true>(P, local, lds); break;
         }
-        __syncthreads();  // the next tile reuses the LDS stages
+        __syncthreads();  // the next tile number and stages reuse the LDS
     }
 }
 
@@ -118,13 +134,10 @@ int grouped_block(int w)
 void *grouped_kernel_ptr(int w)
 {
     switch (w) {
-    case 0: return reinterpret_cast<void *>(&grouped_kernel<float, 128, -1>);
-    case 1: return reinterpret_cast<void *>(&grouped_kernel<float, 64, -1>);
-    case 2: return reinterpret_cast<void *>(&grouped_kernel<double, 64, -1>);
-    case 3: return reinterpret_cast<void *>(&grouped_kernel<double, 128, 0>);
-    case 4: return reinterpret_cast<void *>(&grouped_kernel<double, 128, 1>);
-    case 5: return reinterpret_cast<void *>(&grouped_kernel<double, 128, 2>);
-    default: return reinterpret_cast<void *>(&grouped_kernel<double, 128, 3>);
+    case 0: return reinterpret_cast<void *>(&grouped_kernel<float, 128>);
+    case 1: return reinterpret_cast<void *>(&grouped_kernel<float, 64>);
+    case 2: return reinterpret_cast<void *>(&grouped_kernel<double, 64>);
+    default: return reinterpret_cast<void *>(&grouped_kernel<double, 128>);
     }
 }
 
@@ -150,6 +163,7 @@ int64_t build_group(const std::vector<GroupProblem> (&lists)[kGroupLists], Group
         }
         hdr->count[w] = (int)v.size();
         hdr->total_tiles[w] = tiles;
+        hdr->next[w] = 0;
         tiles_all += tiles;
     }
     return tiles_all;

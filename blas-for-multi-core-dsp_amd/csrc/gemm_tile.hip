@@ -63,10 +63,16 @@ struct PerCall {
     using Cf = Cfg<T, BM, NW>;
 };
 
-template <typename T, int BM, bool TA, bool TB>
+// BODY: kMixed, the unchecked body for interior tiles whose K range is a
+// whole number of K tiles and the checked body elsewhere; kFull, every tile
+// takes the unchecked body (the host has checked that each does); kPtr,
+// 64-bit operand addressing, for leading dimensions too large for the buffer
+// loads' 32-bit offsets (tile::full_offsets_fit).
+enum { kMixed = 0, kFull = 1, kPtr = 2 };
+template <typename T, int BM, bool TA, bool TB, int BODY>
 __global__ void __launch_bounds__((PerCall<T, BM>::Cf::THREADS))
 __attribute__((amdgpu_waves_per_eu(PerCall<T, BM>::Cf::WPE, PerCall<T, BM>::Cf::WPE)))
-tile_gemm_kernel(Gemm<T> g, int tiles_m, int tiles_n, bool a_vec, bool b_vec, bool full_ok, int64_t kslice, T *W)
+tile_gemm_kernel(Gemm<T> g, int tiles_m, int tiles_n, int64_t kslice, T *W)
 {
     using Cf = typename PerCall<T, BM>::Cf;
     __shared__ __attribute__((aligned(16))) T lds[Cf::LDS_ELEMS];
@@ -76,10 +82,12 @@ tile_gemm_kernel(Gemm<T> g, int tiles_m, int tiles_n, bool a_vec, bool b_vec, bo
     // split-K: slice blockIdx.y of K writes its partial to W (one M x N per slice)
     const int64_t kb = (int64_t)blockIdx.y * kslice, ke = min(g.K, kb + kslice);
     T *w = W ? W + (int64_t)blockIdx.y * g.M * g.N : nullptr;
-    if (full_ok && tile::full_tile<Cf>(g, m0, n0, kb, ke, a_vec, b_vec))
-        tile::tile_body<T, BM, TA, TB, Cf::NW, true>(g, m0, n0, a_vec, b_vec, lds, kb, ke, w);
+    if (BODY == kPtr)
+        tile::tile_body<T, BM, TA, TB, Cf::NW, false, true>(g, m0, n0, lds, kb, ke, w);
+    else if (BODY == kFull || tile::full_tile<Cf>(g, m0, n0, kb, ke))
+        tile::tile_body<T, BM, TA, TB, Cf::NW, true>(g, m0, n0, lds, kb, ke, w);
     else
-        tile::tile_body<T, BM, TA, TB, Cf::NW, false>(g, m0, n0, a_vec, b_vec, lds, kb, ke, w);
+        tile::tile_body<T, BM, TA, TB, Cf::NW, false>(g, m0, n0, lds, kb, ke, w);
 }
 
 // C = alpha * sum_s W[s] + beta * C, slices added in order s = 0, 1, ...
@@ -137,24 +145,24 @@ hipError_t launch_tile_bm(const Gemm<T> &g, int splitk, T *work, hipStream_t s)
     const int64_t slices = splitk > 1 ? (g.K + kslice - 1) / kslice : 1;
     if (slices > 1 && !work) return hipErrorInvalidValue;
     T *W = slices > 1 ? work : nullptr;
-    const bool a_vec = tile::vec_ok(g.A, g.lda);
-    const bool b_vec = tile::vec_ok(g.B, g.ldb);
-    // the unchecked body addresses its loads by 32-bit lane offsets
-    const bool full_ok = tile::full_offsets_fit<T, BM>(g.lda) && tile::full_offsets_fit<T, BM>(g.ldb);
+    // the buffer loads address operands by 32-bit lane offsets
+    const bool ptr = !(tile::full_offsets_fit<T, BM>(g.lda) && tile::full_offsets_fit<T, BM>(g.ldb));
     const dim3 grid((unsigned)nwg, (unsigned)slices), block(PerCall<T, BM>::Cf::THREADS);
     const int tmi = (int)tiles_m, tni = (int)tiles_n;
-    if (!g.ta && !g.tb)
-        hipLaunchKernelGGL((tile_gemm_kernel<T, BM, false, false>), grid, block, 0, s, g, tmi, tni, a_vec, b_vec,
-                           full_ok, kslice, W);
-    else if (!g.ta && g.tb)
-        hipLaunchKernelGGL((tile_gemm_kernel<T, BM, false, true>), grid, block, 0, s, g, tmi, tni, a_vec, b_vec,
-                           full_ok, kslice, W);
-    else if (g.ta && !g.tb)
-        hipLaunchKernelGGL((tile_gemm_kernel<T, BM, true, false>), grid, block, 0, s, g, tmi, tni, a_vec, b_vec,
-                           full_ok, kslice, W);
-    else
-        hipLaunchKernelGGL((tile_gemm_kernel<T, BM, true, true>), grid, block, 0, s, g, tmi, tni, a_vec, b_vec,
-                           full_ok, kslice, W);
+    // every tile inside C and every K slice a whole number of K tiles
+    constexpr int64_t BK = PerCall<T, BM>::Cf::BK;
+    const bool all_full = g.M % BM == 0 && g.N % BM == 0 && kslice % BK == 0 && g.K % BK == 0;
+    const int body = ptr ? kPtr : all_full ? kFull : kMixed;
+    const int mode = (g.ta ? 2 : 0) | (g.tb ? 1 : 0);
+    using K = void (*)(Gemm<T>, int, int, int64_t, T *);
+    static constexpr K kernels[3][4] = {
+        {tile_gemm_kernel<T, BM, false, false, kMixed>, tile_gemm_kernel<T, BM, false, true, kMixed>,
+         tile_gemm_kernel<T, BM, true, false, kMixed>, tile_gemm_kernel<T, BM, true, true, kMixed>},
+        {tile_gemm_kernel<T, BM, false, false, kFull>, tile_gemm_kernel<T, BM, false, true, kFull>,
+         tile_gemm_kernel<T, BM, true, false, kFull>, tile_gemm_kernel<T, BM, true, true, kFull>},
+        {tile_gemm_kernel<T, BM, false, false, kPtr>, tile_gemm_kernel<T, BM, false, true, kPtr>,
+         tile_gemm_kernel<T, BM, true, false, kPtr>, tile_gemm_kernel<T, BM, true, true, kPtr>}};
+    hipLaunchKernelGGL(kernels[body][mode], grid, block, 0, s, g, tmi, tni, kslice, W);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || slices == 1) return e;
     const int64_t total = g.M * g.N;

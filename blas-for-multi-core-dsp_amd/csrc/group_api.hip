@@ -1,12 +1,14 @@
 // group_api.hip — C ABI of the grouped persistent GEMM and its hipGraph plan.
 //
 // A plan owns the device problem list, a pinned staging copy of it and an
-// instantiated graph of two kernel nodes (the FP32 and FP64 grouped kernels,
-// in that order).  Running a plan validates the descriptors, builds the
-// sorted tile list in the staging buffer, uploads it with one async copy on
-// the caller's stream and replays the graph on that stream.  The staging
-// buffer is reused only after the previous upload has completed (an event
-// recorded after the copy).
+// instantiated graph of one kernel node per problem list (no edges between
+// them), each at its full persistent grid.  The instantiated graph is never
+// changed after instantiation: running a plan validates the descriptors,
+// builds the sorted tile list (tile counters at zero) in the staging buffer,
+// uploads it with one async copy on the caller's stream and replays the graph
+// on that stream; the kernel of an empty list takes no tile and exits.  The
+// staging buffer is reused only after the previous upload has completed (an
+// event recorded after the copy).
 #include <stdlib.h>
 #include <string.h>
 
@@ -30,18 +32,18 @@ struct blasdsp_group_plan {
     hipEvent_t uploaded = nullptr;  // the staging copy has been read
     hipEvent_t done = nullptr;      // the last graph replay has finished
     bool pending_upload = false;
+    bool has_run = false;           // `done` has been recorded
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipGraphNode_t nodes[blasdsp::kGroupLists] = {};
     int full_grid[blasdsp::kGroupLists] = {};
-    int cur_grid[blasdsp::kGroupLists] = {};
-    bool enabled[blasdsp::kGroupLists] = {};
 };
 
 namespace blasdsp {
 namespace {
 
 constexpr size_t kHeaderSlot = 256;
+static_assert(sizeof(GroupHeader) <= kHeaderSlot, "header fits its slot");
 
 GroupHeader *hdr_of(char *base) { return reinterpret_cast<GroupHeader *>(base); }
 GroupProblem *probs_of(char *base) { return reinterpret_cast<GroupProblem *>(base + kHeaderSlot); }
@@ -49,7 +51,7 @@ GroupProblem *probs_of(char *base) { return reinterpret_cast<GroupProblem *>(bas
 // Kernel node parameters of list w.  The argument values are copied when the
 // node is added or updated, so the locals behind kernelParams may go away.
 struct NodeArgs {
-    const GroupHeader *hdr;
+    GroupHeader *hdr;
     const GroupProblem *probs;
     void *ptrs[2];
 };
@@ -102,12 +104,10 @@ int create_plan(int capacity, blasdsp_group_plan **out)
     }
     memset(p->h_list, 0, p->list_bytes);
     // One kernel node per problem list over one device list; with no edges
-    // between them they may run concurrently.  Nodes of empty lists are
-    // disabled at run time and each node's grid is trimmed to its tile count.
+    // between them they run concurrently.
     const int cus = num_cus();
     for (int w = 0; w < kGroupLists; ++w) {
-        p->full_grid[w] = p->cur_grid[w] = grouped_grid(w, cus);
-        p->enabled[w] = true;
+        p->full_grid[w] = grouped_grid(w, cus);
         hipKernelNodeParams kp = node_params(p, w, p->full_grid[w]);
         e = hipGraphAddKernelNode(&p->nodes[w], p->graph, nullptr, 0, &kp);
         if (e != hipSuccess) {
@@ -139,8 +139,11 @@ int group_tile(int64_t M, int64_t N)
     return c128 <= c64 ? 128 : 64;
 }
 
+// direct: the problem's leading dimensions are too large for the grouped
+// kernel's 32-bit load offsets (tile::full_offsets_fit); it runs through the
+// per-call tile kernel on the plan's stream instead (same C bits).
 template <typename T>
-int to_problem(const blasdsp_gemm_desc &d, GroupProblem &p, bool &skip)
+int to_problem(const blasdsp_gemm_desc &d, GroupProblem &p, bool &skip, bool &direct)
 {
     Gemm<T> g;
     int st = check<T>(g, d.transA, d.transB, d.M, d.N, d.K, (T)d.alpha, (const T *)d.A, d.lda,
@@ -154,14 +157,26 @@ int to_problem(const blasdsp_gemm_desc &d, GroupProblem &p, bool &skip)
     p.lda = d.lda; p.ldb = d.ldb; p.ldc = d.ldc;
     p.alpha = d.alpha; p.beta = d.beta;
     p.ta = g.ta; p.tb = g.tb;
-    p.a_vec = tile::vec_ok((const T *)d.A, d.lda);
-    p.b_vec = tile::vec_ok((const T *)d.B, d.ldb);
     TuneChoice c;
     if (tune_lookup(sizeof(T) == 8, g.ta, g.tb, d.M, d.N, d.K, &c) && c.algo == BLASDSP_ALGO_TILE)
         p.bm = c.tile;
     else
         p.bm = group_tile(d.M, d.N);
+    direct = !(tile::full_offsets_fit<T, 128>(d.lda) && tile::full_offsets_fit<T, 128>(d.ldb));
     return BLASDSP_SUCCESS;
+}
+
+template <typename T>
+hipError_t run_direct(const GroupProblem &p, hipStream_t s)
+{
+    Gemm<T> g;
+    g.ta = p.ta; g.tb = p.tb;
+    g.M = p.M; g.N = p.N; g.K = p.K;
+    g.alpha = (T)p.alpha; g.beta = (T)p.beta;
+    g.A = (const T *)p.A; g.lda = p.lda;
+    g.B = (const T *)p.B; g.ldb = p.ldb;
+    g.C = (T *)p.C; g.ldc = p.ldc;
+    return launch_tile<T>(g, p.bm, 1, nullptr, s);
 }
 
 int run_plan(blasdsp_group_plan *p, const blasdsp_gemm_desc *descs, int count, hipStream_t s)
@@ -169,18 +184,22 @@ int run_plan(blasdsp_group_plan *p, const blasdsp_gemm_desc *descs, int count, h
     if (!p || count < 0 || (count > 0 && !descs)) return BLASDSP_ERR_INVALID;
     if (count > p->capacity) return BLASDSP_ERR_INVALID;
     std::vector<GroupProblem> lists[kGroupLists];
+    std::vector<std::pair<bool, GroupProblem>> direct_probs;
     int used = 0;
     for (int i = 0; i < count; ++i) {
         GroupProblem gp;
-        bool skip = false;
+        bool skip = false, direct = false;
         int st;
         const bool f64 = descs[i].dtype == BLASDSP_FLOAT64;
-        if (descs[i].dtype == BLASDSP_FLOAT32) st = to_problem<float>(descs[i], gp, skip);
-        else if (f64) st = to_problem<double>(descs[i], gp, skip);
+        if (descs[i].dtype == BLASDSP_FLOAT32) st = to_problem<float>(descs[i], gp, skip, direct);
+        else if (f64) st = to_problem<double>(descs[i], gp, skip, direct);
         else st = BLASDSP_ERR_INVALID;
         if (st != BLASDSP_SUCCESS) return st;
-        if (!skip) {
-            lists[group_list(f64, gp.bm, (gp.ta << 1) | gp.tb)].push_back(gp);
+        if (skip) continue;
+        if (direct) {
+            direct_probs.emplace_back(f64, gp);
+        } else {
+            lists[group_list(f64, gp.bm)].push_back(gp);
             ++used;
         }
     }
@@ -193,29 +212,19 @@ int run_plan(blasdsp_group_plan *p, const blasdsp_gemm_desc *descs, int count, h
     GroupHeader *h = hdr_of(p->h_list);
     if (build_group(lists, h, probs_of(p->h_list), p->capacity) < 0) return BLASDSP_ERR_INVALID;
     const size_t bytes = kHeaderSlot + (size_t)used * sizeof(GroupProblem);
-    // Enable the nodes with work, each with no more workgroups than tiles.
-    for (int w = 0; w < kGroupLists; ++w) {
-        const int64_t tiles = h->total_tiles[w];
-        const bool on = tiles > 0;
-        const int grid = (int)(tiles < p->full_grid[w] ? (tiles > 0 ? tiles : 1) : p->full_grid[w]);
-        hipError_t e = hipSuccess;
-        if (on && grid != p->cur_grid[w]) {
-            hipKernelNodeParams kp = node_params(p, w, grid);
-            e = hipGraphExecKernelNodeSetParams(p->exec, p->nodes[w], &kp);
-            if (e == hipSuccess) p->cur_grid[w] = grid;
-        }
-        if (e == hipSuccess && on != p->enabled[w]) {
-            e = hipGraphNodeSetEnabled(p->exec, p->nodes[w], on ? 1 : 0);
-            if (e == hipSuccess) p->enabled[w] = on;
-        }
-        if (e != hipSuccess) return hip_status(e);
-    }
-    hipError_t e = hipMemcpyAsync(p->d_list, p->h_list, bytes, hipMemcpyHostToDevice, s);
+    // The previous replay may still be reading the device list and counters:
+    // order the copy after it (on the same stream this wait is already met).
+    hipError_t e = p->has_run ? hipStreamWaitEvent(s, p->done, 0) : hipSuccess;
+    if (e == hipSuccess) e = hipMemcpyAsync(p->d_list, p->h_list, bytes, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipEventRecord(p->uploaded, s);
     if (e != hipSuccess) return hip_status(e);
     p->pending_upload = true;
     e = hipGraphLaunch(p->exec, s);
+    for (size_t i = 0; i < direct_probs.size() && e == hipSuccess; ++i)
+        e = direct_probs[i].first ? run_direct<double>(direct_probs[i].second, s)
+                                  : run_direct<float>(direct_probs[i].second, s);
     if (e == hipSuccess) e = hipEventRecord(p->done, s);
+    if (e == hipSuccess) p->has_run = true;
     return hip_status(e);
 }
 

@@ -19,37 +19,42 @@ struct GroupProblem {
     int64_t tile_begin;   // first tile of this problem in its precision's list
     int64_t tiles_n;
     int32_t bm;           // tile edge, 64 or 128
-    uint8_t ta, tb, a_vec, b_vec;
+    uint8_t ta, tb;
 };
 
-// Seven problem lists, one per grouped kernel:
-//   0: FP32, 128 tile    1: FP32, 64 tile    2: FP64, 64 tile   (all modes)
-//   3 + mode: FP64, 128 tile, one list per transpose mode (ta * 2 + tb)
-// The FP64 128 tile body needs ~240 VGPRs; folding its four modes into one
-// kernel pushes it past 256 and spills, so that case gets a kernel per mode.
+// Four problem lists, one per grouped kernel, each holding every transpose
+// mode: 0: FP32, 128 tile   1: FP32, 64 tile   2: FP64, 64 tile
+// 3: FP64, 128 tile.  A process gets four hardware queues, so the four kernel
+// nodes of a group's graph all start together.
 // Problems of list w are probs[first[w] .. first[w] + count[w]).
-constexpr int kGroupLists = 7;
-__host__ __device__ inline int group_list(bool f64, int bm, int mode)
+constexpr int kGroupLists = 4;
+__host__ __device__ inline int group_list(bool f64, int bm)
 {
     if (!f64) return bm == 64 ? 1 : 0;
-    return bm == 64 ? 2 : 3 + mode;
+    return bm == 64 ? 2 : 3;
 }
 
+// next[w]: the tile counter of list w.  Each workgroup of list w's kernel
+// takes its next tile with an atomic increment, so a workgroup that starts
+// late (its slot held by another list's kernel) takes what is left instead of
+// a fixed share.  The host uploads the header with every counter at zero
+// before each replay.
 struct GroupHeader {
     int32_t count[kGroupLists];
     int32_t first[kGroupLists];
     int64_t total_tiles[kGroupLists];
+    unsigned long long next[kGroupLists];
 };
 
 // Persistent grid of list w's kernel (its occupancy times the CU count).
 int grouped_grid(int w, int num_cus);
 // Threads per workgroup of list w's kernel.
 int grouped_block(int w);
-// Workgroups of one list's kernel that can hold a tile at once, for sizing
-// the grid of a small list.
+// Kernel of list w.
 void *grouped_kernel_ptr(int w);
-// Sorts each list by decreasing K, assigns tile ranges and writes header and
-// problems; returns the total tile count, or -1 past `capacity` problems.
+// Sorts each list by decreasing K, assigns tile ranges and writes header
+// (counters at zero) and problems; returns the total tile count, or -1 past
+// `capacity` problems.
 int64_t build_group(const std::vector<GroupProblem> (&lists)[kGroupLists], GroupHeader *hdr,
                     GroupProblem *out, int capacity);
 

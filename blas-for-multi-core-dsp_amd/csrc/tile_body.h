@@ -23,8 +23,11 @@
 //     MFMAs consume tile k; one barrier per K tile;
 //   * epilogue C = alpha * acc + beta * C fused into the store (C is not read
 //     when beta == 0);
-//   * interior tiles of aligned operands with K a multiple of BK take a
-//     specialised body with no bounds checks (5 % faster at FP64 4096^3).
+//   * operand loads are buffer loads at any element alignment of the leading
+//     dimension; out-of-range rows and the ragged edge read as zero through
+//     the buffer's range check (see Stager::load);
+//   * interior tiles with K a multiple of BK take a specialised body with no
+//     bounds checks (5 % faster at FP64 4096^3).
 #pragma once
 
 #include "internal.h"
@@ -101,51 +104,80 @@ struct Stager {
 
     vec_t r[NCH];
 
-    // X: stored matrix with R rows, Cn columns, leading dimension ld; the tile
-    // starts at stored (r0, c0).  Out-of-range elements read as zero.  FULL:
-    // the caller guarantees the whole tile is in range, vector-aligned and
-    // within full_offsets_fit(); the loads are then buffer loads from the tile's
-    // workgroup-uniform origin (a scalar resource) plus one 32-bit lane
-    // offset, which keeps 64-bit per-lane addresses out of the registers.
-    template <bool FULL>
-    __device__ __forceinline__ void load(const T *__restrict__ X, int64_t ld, int64_t R, int64_t Cn,
-                                         int64_t r0, int64_t c0, bool vec, int tid)
+    // One element of a buffer load; an offset at or past the resource's
+    // extent (0x7fffffff bytes) reads as zero without accessing memory.
+    static __device__ __forceinline__ T load_elem(__amdgpu_buffer_rsrc_t rs, uint32_t off)
     {
-        if (FULL) {
-            // vector i: row tid / CPR + i * (THREADS / CPR), column (tid % CPR) * V
-            constexpr int RSTEP = Cf::THREADS / CPR;
-            static_assert(Cf::THREADS % CPR == 0, "whole rows per step");
-            const uint32_t off = (uint32_t)(((int64_t)(tid / CPR) * ld + (tid % CPR) * Cf::V) * (int64_t)sizeof(T));
+        if constexpr (sizeof(T) == 8)
+            return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
+        else
+            return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
+    }
+
+    // X: stored matrix with R rows, Cn columns, leading dimension ld; the tile
+    // starts at stored (r0, c0), r0 < R and c0 < Cn (or Cn == 0: nothing is
+    // read).  Elements outside the matrix read as zero.  Thread tid's vector
+    // i is row tid / CPR + i * RSTEP, columns (tid % CPR) * V + [0, V).
+    //
+    // PTR = false (the leading dimension within full_offsets_fit): buffer
+    // loads from each vector row's workgroup-uniform origin (a scalar
+    // resource) plus one 32-bit lane offset.  When the tile's columns are
+    // inside the matrix each 16-byte vector is one load at any element
+    // alignment of ld (the HSA code object model enables unaligned global
+    // access; the compiler itself lowers align-4 16-byte loads to the same
+    // instruction), and a row past R gets an offset past the resource's extent,
+    // which reads as zero.  When the tile's columns reach past Cn (the K tail
+    // of a K-contiguous operand, the M or N edge of an M/N-contiguous one) the
+    // vectors are loaded element by element, each element outside with such
+    // an offset.  FULL: the caller guarantees the whole tile is inside.
+    // PTR = true: 64-bit per-lane addresses, for leading dimensions whose row
+    // offsets do not fit 32 bits.
+    template <bool FULL, bool PTR = false>
+    __device__ __forceinline__ void load(const T *__restrict__ X, int64_t ld, int64_t R, int64_t Cn,
+                                         int64_t r0, int64_t c0, int tid)
+    {
+        constexpr int RSTEP = Cf::THREADS / CPR;
+        static_assert(Cf::THREADS % CPR == 0, "whole rows per step");
+        const int row = tid / CPR, col = (tid % CPR) * Cf::V;
+        const bool cols_in = FULL || c0 + TC <= Cn;
+        if constexpr (!PTR) {
+            constexpr uint32_t kOut = 0x80000000u;
+            const uint32_t off = (uint32_t)(((int64_t)row * ld + col) * (int64_t)sizeof(T));
+            // resource of vector i: its row origin (r0 + i RSTEP, c0)
+            auto rsrc = [&](int i) {
+                const uintptr_t base =
+                    (uintptr_t)X + (uintptr_t)(((r0 + (int64_t)i * RSTEP) * ld + c0) * (int64_t)sizeof(T));
+                return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, 0x7fffffff, 0x00020000);
+            };
+            if (cols_in) {
 #pragma unroll
-            for (int i = 0; i < NCH; ++i) {
-                const T *base = X + (r0 + (int64_t)i * RSTEP) * ld + c0;
-                const __amdgpu_buffer_rsrc_t rs =
-                    __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(base), (short)0, 0x7fffffff, 0x00020000);
-                r[i] = __builtin_bit_cast(vec_t, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+                for (int i = 0; i < NCH; ++i) {
+                    const bool rok = FULL || r0 + row + (int64_t)i * RSTEP < R;
+                    r[i] = __builtin_bit_cast(vec_t,
+                                              __builtin_amdgcn_raw_buffer_load_b128(rsrc(i), rok ? off : kOut, 0, 0));
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < NCH; ++i) {
+                    const __amdgpu_buffer_rsrc_t rs = rsrc(i);
+                    const bool rok = r0 + row + (int64_t)i * RSTEP < R;
+#pragma unroll
+                    for (int e = 0; e < Cf::V; ++e) {
+                        const bool ok = rok && c0 + col + e < Cn;
+                        r[i][e] = load_elem(rs, ok ? off + (uint32_t)(e * (int)sizeof(T)) : kOut);
+                    }
+                }
             }
             return;
         }
-        const bool full = vec && (r0 + TR <= R) && (c0 + TC <= Cn);
-        if (full) {
 #pragma unroll
-            for (int i = 0; i < NCH; ++i) {
-                const int c = tid + i * Cf::THREADS;
-                const int row = c / CPR, col = (c % CPR) * Cf::V;
-                r[i] = *reinterpret_cast<const vec_t *>(X + (r0 + row) * ld + c0 + col);
-            }
-        } else {
+        for (int i = 0; i < NCH; ++i) {
+            const int64_t gr = r0 + row + (int64_t)i * RSTEP, gc = c0 + col;
+            if (cols_in && gr < R) {
+                __builtin_memcpy(&r[i], X + gr * ld + gc, sizeof(vec_t));  // 16 bytes at element alignment
+            } else {
 #pragma unroll
-            for (int i = 0; i < NCH; ++i) {
-                const int c = tid + i * Cf::THREADS;
-                const int row = c / CPR, col = (c % CPR) * Cf::V;
-                const int64_t gr = r0 + row, gc = c0 + col;
-                if (vec && gr < R && gc + Cf::V <= Cn) {
-                    r[i] = *reinterpret_cast<const vec_t *>(X + gr * ld + gc);
-                } else {
-#pragma unroll
-                    for (int e = 0; e < Cf::V; ++e)
-                        r[i][e] = (gr < R && gc + e < Cn) ? X[gr * ld + gc + e] : T(0);
-                }
+                for (int e = 0; e < Cf::V; ++e) r[i][e] = (gr < R && gc + e < Cn) ? X[gr * ld + gc + e] : T(0);
             }
         }
     }
@@ -169,14 +201,12 @@ inline bool full_offsets_fit(int64_t ld)
     return (double)(BM - 1) * (double)ld * sizeof(T) + 4096.0 < 2147483647.0;
 }
 
-// True when tile (m0, n0) over K range [kb, ke) can take the FULL body: both
-// operands vector-aligned, the tile inside C and the range a whole number of
-// K tiles.
+// True when tile (m0, n0) over K range [kb, ke) can take the FULL body: the
+// tile inside C and the range a whole number of K tiles.
 template <typename Cf, typename T>
-__device__ __forceinline__ bool full_tile(const Gemm<T> &g, int64_t m0, int64_t n0, int64_t kb, int64_t ke,
-                                          bool a_vec, bool b_vec)
+__device__ __forceinline__ bool full_tile(const Gemm<T> &g, int64_t m0, int64_t n0, int64_t kb, int64_t ke)
 {
-    return a_vec && b_vec && m0 + Cf::BM <= g.M && n0 + Cf::BM <= g.N && (ke - kb) % Cf::BK == 0;
+    return m0 + Cf::BM <= g.M && n0 + Cf::BM <= g.N && (ke - kb) % Cf::BK == 0;
 }
 
 // Wave issue priority (s_setprio 1) over one section of each K tile, per
@@ -201,13 +231,14 @@ constexpr bool store_prio()
 // One BM x BM tile of C at (m0, n0) with NW waves, over K range [kb, ke)
 // (kb a multiple of BK).  lds: Cfg::LDS_ELEMS elements.  Every wave of the
 // workgroup must call it (it contains barriers).  FULL (see full_tile): no
-// bounds checks anywhere.  W == nullptr: C = alpha acc + beta C; otherwise
-// the raw sums go to W (M x N, leading dimension N), a split-K partial.  The
-// MFMA sequence of every C element is the same for any NW and FULL, so those
-// variants give bit-identical results.
-template <typename T, int BM, bool TA, bool TB, int NW = 4, bool FULL = false>
-__device__ __forceinline__ void tile_body(const Gemm<T> &g, int64_t m0, int64_t n0, bool a_vec,
-                                          bool b_vec, T *lds, int64_t kb = 0, int64_t ke = -1, T *W = nullptr)
+// bounds checks anywhere.  PTR: 64-bit operand addressing (both leading
+// dimensions must pass full_offsets_fit otherwise).  W == nullptr: C = alpha
+// acc + beta C; otherwise the raw sums go to W (M x N, leading dimension N), a
+// split-K partial.  The MFMA sequence of every C element is the same for any
+// NW, FULL and PTR, so those variants give bit-identical results.
+template <typename T, int BM, bool TA, bool TB, int NW = 4, bool FULL = false, bool PTR = false>
+__device__ __forceinline__ void tile_body(const Gemm<T> &g, int64_t m0, int64_t n0, T *lds, int64_t kb = 0,
+                                          int64_t ke = -1, T *W = nullptr)
 {
     if (ke < 0) ke = g.K;
     using Cf = Cfg<T, BM, NW>;
@@ -227,10 +258,10 @@ __device__ __forceinline__ void tile_body(const Gemm<T> &g, int64_t m0, int64_t 
     Stager<Cf, T, !TA> sa;
     Stager<Cf, T, TB> sb;
     auto load_tiles = [&](int64_t k0) {
-        if (TA) sa.template load<FULL>(g.A, g.lda, AR, AC, k0, m0, a_vec, tid);
-        else    sa.template load<FULL>(g.A, g.lda, AR, AC, m0, k0, a_vec, tid);
-        if (TB) sb.template load<FULL>(g.B, g.ldb, BR, BC, n0, k0, b_vec, tid);
-        else    sb.template load<FULL>(g.B, g.ldb, BR, BC, k0, n0, b_vec, tid);
+        if (TA) sa.template load<FULL, PTR>(g.A, g.lda, AR, AC, k0, m0, tid);
+        else    sa.template load<FULL, PTR>(g.A, g.lda, AR, AC, m0, k0, tid);
+        if (TB) sb.template load<FULL, PTR>(g.B, g.ldb, BR, BC, n0, k0, tid);
+        else    sb.template load<FULL, PTR>(g.B, g.ldb, BR, BC, k0, n0, tid);
     };
     auto stage = [&](int st, int op) { return lds + (st * 2 + op) * BUF; };
 
@@ -306,14 +337,6 @@ __device__ __forceinline__ void tile_body(const Gemm<T> &g, int64_t m0, int64_t 
             }
         }
     }
-}
-
-// Vector loads are used when the base is 16-byte aligned and every row starts
-// on a 16-byte boundary.
-template <typename T>
-inline bool vec_ok(const T *p, int64_t ld)
-{
-    return ((uintptr_t)p % 16 == 0) && (ld % (16 / (int64_t)sizeof(T)) == 0);
 }
 
 }  // namespace tile

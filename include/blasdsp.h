@@ -321,7 +321,8 @@ BLASDSP_API int blasdsp_gemm_grouped(const blasdsp_gemm_desc *descs, int count, 
 
 /* A reusable plan: device problem list, pinned staging and an instantiated
  * hipGraph of the grouped kernels.  run() uploads the list and replays the
- * graph on `stream`.  A plan is used by one host thread at a time. */
+ * graph on `stream`; a run on another stream than the previous one waits for
+ * the previous replay first.  A plan is used by one host thread at a time. */
 typedef struct blasdsp_group_plan blasdsp_group_plan;
 BLASDSP_API int blasdsp_group_plan_create(int max_problems, blasdsp_group_plan **plan);
 BLASDSP_API int blasdsp_group_plan_run(blasdsp_group_plan *plan, const blasdsp_gemm_desc *descs,

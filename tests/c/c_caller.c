@@ -5,13 +5,14 @@
  * Built with gcc against include/blasdsp.h and linked to libblasdsp.so and
  * the system HIP runtime (no Python, no torch in the process), so it checks
  * the C boundary as a C program sees it.  The serving-loop check is present
- * but switched off (see main()).  Results are compared with the CPU
+ * but switched off (see main()); the program prints a SKIPPED line for it.
+ * Results are compared with the CPU
  * oracle (oracle/liboracle.so): normwise error <= 1e-13 for FP64,
  * (sqrt(K) + 4) 2^-23 for FP32, 1e-6 for the split-FP16 form, 2^-11 for the
  * FP8 form against the oracle's quantized product, and exact for the integer
  * known-answer case and the uniform generator.
  *
- * Prints one line per check and exits 0 only if every check passed. */
+ * Prints one line per check and exits 0 only if every check that ran passed. */
 #define __HIP_PLATFORM_AMD__ 1
 #include <hip/hip_runtime_api.h>
 #include <math.h>
@@ -500,6 +501,7 @@ int main(void)
      * no cause has been shown.  The serving loop keeps its tests in
      * tests/test_serve_gpu.py. */
     (void)check_serve;
+    printf("SKIPPED blasdsp_serve grouped and per-request (40): switched off, no cause shown for one abort\n");
     printf("%s\n", g_failed ? "FAILED" : "PASSED");
     return g_failed ? 1 : 0;
 }

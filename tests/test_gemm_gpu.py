@@ -87,6 +87,27 @@ def test_skinny_kernel_modes(orc, blas, dev, prec, mode, shape, pad):
 
 @pytest.mark.parametrize("prec", ["d", "s"])
 @pytest.mark.parametrize("mode", ["NN", "NT", "TN", "TT"])
+@pytest.mark.parametrize("shape", [(300, 260, 256), (513, 257, 320), (256, 384, 197)])
+def test_tile_alignment_invariance(orc, blas, dev, prec, mode, shape):
+    """The tile kernel with operand rows 16-byte aligned (pad 4) and not (pads
+    1 and 3): a vector is one 16-byte load at any element alignment, edges and
+    the K tail read as zero through the buffer range check, so C is bit for bit
+    the same, and within the bar of the oracle.  K = 256 and 320 give interior
+    tiles a whole number of K tiles (the unchecked body), K = 197 a K tail."""
+    M, N, K = shape
+    outs = []
+    for pad in (4, 1, 3):
+        got, ref = run_case(orc, blas, dev, prec, mode[0], mode[1], M, N, K, 1.5, -0.5, blas.ALGO_TILE,
+                            splitk=1, pad=pad, cpad=pad)
+        err, _ = orc.normwise(got, ref)
+        assert err <= (TOL64 if prec == "d" else tol32(K)), (pad, err)
+        outs.append(got)
+    np.testing.assert_array_equal(outs[0], outs[1])
+    np.testing.assert_array_equal(outs[0], outs[2])
+
+
+@pytest.mark.parametrize("prec", ["d", "s"])
+@pytest.mark.parametrize("mode", ["NN", "NT", "TN", "TT"])
 @pytest.mark.parametrize("splitk", [2, 3, 5])
 @pytest.mark.parametrize("shape", [(200, 136, 1000), (257, 129, 1111), (64, 64, 2048), (130, 300, 515)])
 def test_tile_splitk(orc, blas, dev, prec, mode, splitk, shape):

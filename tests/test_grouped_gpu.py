@@ -83,6 +83,83 @@ def test_group_plan_replays(orc, blas, dev):
         plan.close()
 
 
+def test_group_plan_two_streams_and_large_tiles(orc, blas, dev):
+    """One plan run on two streams in turn (the second run is ordered after
+    the first replay), on groups of FP32 and FP64 problems of every mode that
+    take 128 tiles (M, N in 300..700), each bit for bit against the problem
+    run alone; then a group of empty problems only."""
+    import torch
+
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    plan = blas.GroupPlan(16)
+    try:
+        groups = []
+        for g, stream in enumerate((s1, s2, s1)):
+            rng = np.random.default_rng(40 + g)
+            probs = []
+            for i in range(8):
+                dt = np.float64 if i % 2 else np.float32
+                mode = ["NN", "NT", "TN", "TT"][i % 4]
+                M, N, K = (int(x) for x in rng.integers(300, 700, 3))
+                A = orc.matrix(M, K, 0, 500 + 10 * g + i, dtype=dt, trans=mode[0] == "T")
+                B = orc.matrix(K, N, 0, 600 + 10 * g + i, dtype=dt, trans=mode[1] == "T")
+                C0 = orc.matrix(M, N, 0, 700 + 10 * g + i, dtype=dt)
+                probs.append(dict(mode=mode, A=to_dev(A, dev), B=to_dev(B, dev), C0=to_dev(C0, dev)))
+            outs = [p["C0"].clone() for p in probs]
+            torch.cuda.current_stream().synchronize()
+            descs = [blas.desc(p["mode"][0], p["mode"][1], p["A"], p["B"], c, 1.0, 0.5)
+                     for p, c in zip(probs, outs)]
+            plan.run(descs, stream=stream)
+            groups.append((probs, outs))
+        torch.cuda.synchronize()
+        for g, (probs, outs) in enumerate(groups):
+            for i, (p, c) in enumerate(zip(probs, outs)):
+                alone = p["C0"].clone()
+                blas.gemm(p["mode"][0], p["mode"][1], p["A"], p["B"], alone, 1.0, 0.5, algo=blas.ALGO_TILE,
+                          splitk=1)
+                torch.cuda.synchronize()
+                assert torch.equal(c, alone), (g, i)
+        empty = torch.zeros((0, 5), device=dev)
+        plan.run([blas.desc("N", "N", empty, torch.zeros((5, 7), device=dev), torch.zeros((0, 7), device=dev))])
+        torch.cuda.synchronize()
+    finally:
+        plan.close()
+
+
+def test_group_wide_leading_dimension(orc, blas, dev):
+    """A group holding an FP32 problem whose A rows are 4.4 million elements
+    apart (past the grouped kernel's 32-bit load offsets: it runs through the
+    per-call kernel on the plan's stream) beside an ordinary one; both against
+    the oracle."""
+    import torch
+
+    M, K, N, lda = 3, 5000, 70, 4_400_000
+    Abuf = torch.empty((M, lda), dtype=torch.float32, device=dev)
+    A = Abuf[:, :K]
+    B = torch.empty((K, N), dtype=torch.float32, device=dev)
+    C = torch.empty((M, N), dtype=torch.float32, device=dev)
+    blas.fill(A, M, K, 0, 61)
+    blas.fill(B, K, N, 0, 62)
+    blas.fill(C, M, N, 0, 63)
+    A2 = torch.empty((200, 300), dtype=torch.float32, device=dev)
+    C2 = torch.empty((200, N), dtype=torch.float32, device=dev)
+    blas.fill(A2, 200, 300, 0, 64)
+    blas.fill(C2, 200, N, 0, 65)
+    c0, c20 = C.cpu().numpy().copy(), C2.cpu().numpy().copy()
+    blas.gemm_grouped([blas.desc("N", "N", A, B, C, 1.0, 1.0), blas.desc("N", "N", A2, B[:300], C2, 1.0, 1.0)])
+    torch.cuda.synchronize()
+    a = np.ascontiguousarray(A.cpu().numpy())
+    b = np.ascontiguousarray(B.cpu().numpy())
+    ref = orc.gemm_xp("N", "N", a, b, c0, 1.0, 1.0)
+    err, _ = orc.normwise(np.ascontiguousarray(C.cpu().numpy()), ref)
+    assert err <= (np.sqrt(K) + 4) * 2.0 ** -23, err
+    ref2 = orc.gemm_xp("N", "N", np.ascontiguousarray(A2.cpu().numpy()), b[:300], c20, 1.0, 1.0)
+    err2, _ = orc.normwise(np.ascontiguousarray(C2.cpu().numpy()), ref2)
+    assert err2 <= (np.sqrt(300) + 4) * 2.0 ** -23, err2
+    del Abuf
+    torch.cuda.empty_cache()
+
+
 def test_group_rejects_bad_problem(blas, dev):
     import torch
 
