@@ -66,8 +66,8 @@ struct PerCall {
 // BODY: kMixed, the unchecked body for interior tiles whose K range is a
 // whole number of K tiles and the checked body elsewhere; kFull, every tile
 // takes the unchecked body (the host has checked that each does); kPtr,
-// 64-bit operand addressing, for leading dimensions too large for the buffer
-// loads' 32-bit offsets (tile::full_offsets_fit).
+// 64-bit operand addressing, for operands the buffer loads do not take
+// (tile::buffer_ok).
 enum { kMixed = 0, kFull = 1, kPtr = 2 };
 template <typename T, int BM, bool TA, bool TB, int BODY>
 __global__ void __launch_bounds__((PerCall<T, BM>::Cf::THREADS))
@@ -145,8 +145,7 @@ hipError_t launch_tile_bm(const Gemm<T> &g, int splitk, T *work, hipStream_t s)
     const int64_t slices = splitk > 1 ? (g.K + kslice - 1) / kslice : 1;
     if (slices > 1 && !work) return hipErrorInvalidValue;
     T *W = slices > 1 ? work : nullptr;
-    // the buffer loads address operands by 32-bit lane offsets
-    const bool ptr = !(tile::full_offsets_fit<T, BM>(g.lda) && tile::full_offsets_fit<T, BM>(g.ldb));
+    const bool ptr = !tile::buffer_ok<T, BM>(g);
     const dim3 grid((unsigned)nwg, (unsigned)slices), block(PerCall<T, BM>::Cf::THREADS);
     const int tmi = (int)tiles_m, tni = (int)tiles_n;
     // every tile inside C and every K slice a whole number of K tiles

@@ -139,9 +139,10 @@ int group_tile(int64_t M, int64_t N)
     return c128 <= c64 ? 128 : 64;
 }
 
-// direct: the problem's leading dimensions are too large for the grouped
-// kernel's 32-bit load offsets (tile::full_offsets_fit); it runs through the
-// per-call tile kernel on the plan's stream instead (same C bits).
+// direct: the grouped kernel's buffer loads do not take the problem (leading
+// dimensions past their 32-bit offsets, or a stored operand narrower than one
+// 16-byte vector: tile::buffer_ok); it runs through the per-call tile kernel
+// on the plan's stream instead (same C bits).
 template <typename T>
 int to_problem(const blasdsp_gemm_desc &d, GroupProblem &p, bool &skip, bool &direct)
 {
@@ -162,7 +163,7 @@ int to_problem(const blasdsp_gemm_desc &d, GroupProblem &p, bool &skip, bool &di
         p.bm = c.tile;
     else
         p.bm = group_tile(d.M, d.N);
-    direct = !(tile::full_offsets_fit<T, 128>(d.lda) && tile::full_offsets_fit<T, 128>(d.ldb));
+    direct = !tile::buffer_ok<T, 128>(g);
     return BLASDSP_SUCCESS;
 }
 

@@ -103,6 +103,7 @@ struct Stager {
     static_assert(TR * CPR % Cf::THREADS == 0, "whole vectors per thread");
 
     vec_t r[NCH];
+    int sh = 0;  // the loaded vectors start sh elements early (see load)
 
     // One element of a buffer load; an offset at or past the resource's
     // extent (0x7fffffff bytes) reads as zero without accessing memory.
@@ -119,19 +120,23 @@ struct Stager {
     // read).  Elements outside the matrix read as zero.  Thread tid's vector
     // i is row tid / CPR + i * RSTEP, columns (tid % CPR) * V + [0, V).
     //
-    // PTR = false (the leading dimension within full_offsets_fit): buffer
-    // loads from each vector row's workgroup-uniform origin (a scalar
-    // resource) plus one 32-bit lane offset.  When the tile's columns are
-    // inside the matrix each 16-byte vector is one load at any element
-    // alignment of ld (the HSA code object model enables unaligned global
-    // access; the compiler itself lowers align-4 16-byte loads to the same
-    // instruction), and a row past R gets an offset past the resource's extent,
-    // which reads as zero.  When the tile's columns reach past Cn (the K tail
-    // of a K-contiguous operand, the M or N edge of an M/N-contiguous one) the
-    // vectors are loaded element by element, each element outside with such
-    // an offset.  FULL: the caller guarantees the whole tile is inside.
+    // PTR = false (the leading dimension within full_offsets_fit, Cn >= V):
+    // buffer loads from each vector row's workgroup-uniform origin (a scalar
+    // resource) plus one 32-bit lane offset, one 16-byte load per vector at
+    // any element alignment of ld (the HSA code object model enables
+    // unaligned global access; the compiler itself lowers align-4 16-byte
+    // loads to the same instruction).  A vector whose row is past R or whose
+    // first column is past Cn gets an offset past the resource's extent,
+    // which reads as zero without accessing memory.  A vector that starts
+    // inside and ends past Cn (the K tail of a K-contiguous operand, the M or
+    // N edge of an M/N-contiguous one) is loaded shifted back to end at
+    // column Cn - 1, and store() shifts it into place with zeros behind.  The
+    // code has no branches, so the loads of both operands stay in flight
+    // together (a branch between them made the compiler wait for the first
+    // operand's loads before issuing the second's).  FULL: the caller
+    // guarantees the whole tile is inside.
     // PTR = true: 64-bit per-lane addresses, for leading dimensions whose row
-    // offsets do not fit 32 bits.
+    // offsets do not fit 32 bits and for matrices of fewer than V columns.
     template <bool FULL, bool PTR = false>
     __device__ __forceinline__ void load(const T *__restrict__ X, int64_t ld, int64_t R, int64_t Cn,
                                          int64_t r0, int64_t c0, int tid)
@@ -139,37 +144,30 @@ struct Stager {
         constexpr int RSTEP = Cf::THREADS / CPR;
         static_assert(Cf::THREADS % CPR == 0, "whole rows per step");
         const int row = tid / CPR, col = (tid % CPR) * Cf::V;
-        const bool cols_in = FULL || c0 + TC <= Cn;
         if constexpr (!PTR) {
             constexpr uint32_t kOut = 0x80000000u;
-            const uint32_t off = (uint32_t)(((int64_t)row * ld + col) * (int64_t)sizeof(T));
-            // resource of vector i: its row origin (r0 + i RSTEP, c0)
-            auto rsrc = [&](int i) {
+            const int64_t gc = c0 + col;
+            const bool col_in = FULL || gc < Cn;
+            const int64_t over = gc + Cf::V - Cn;  // elements past the last column
+            sh = (FULL || !col_in || over <= 0) ? 0 : (int)over;
+            // The resource base sits V elements before the vector row's origin
+            // so that a shifted vector's offset (at least V - sh > 0) stays
+            // positive; the addresses read are never before the origin's row.
+            const uint32_t off = (uint32_t)(((int64_t)row * ld + col + Cf::V - sh) * (int64_t)sizeof(T));
+#pragma unroll
+            for (int i = 0; i < NCH; ++i) {
                 const uintptr_t base =
-                    (uintptr_t)X + (uintptr_t)(((r0 + (int64_t)i * RSTEP) * ld + c0) * (int64_t)sizeof(T));
-                return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, 0x7fffffff, 0x00020000);
-            };
-            if (cols_in) {
-#pragma unroll
-                for (int i = 0; i < NCH; ++i) {
-                    const bool rok = FULL || r0 + row + (int64_t)i * RSTEP < R;
-                    r[i] = __builtin_bit_cast(vec_t,
-                                              __builtin_amdgcn_raw_buffer_load_b128(rsrc(i), rok ? off : kOut, 0, 0));
-                }
-            } else {
-#pragma unroll
-                for (int i = 0; i < NCH; ++i) {
-                    const __amdgpu_buffer_rsrc_t rs = rsrc(i);
-                    const bool rok = r0 + row + (int64_t)i * RSTEP < R;
-#pragma unroll
-                    for (int e = 0; e < Cf::V; ++e) {
-                        const bool ok = rok && c0 + col + e < Cn;
-                        r[i][e] = load_elem(rs, ok ? off + (uint32_t)(e * (int)sizeof(T)) : kOut);
-                    }
-                }
+                    (uintptr_t)X +
+                    (uintptr_t)(((r0 + (int64_t)i * RSTEP) * ld + c0 - Cf::V) * (int64_t)sizeof(T));
+                const __amdgpu_buffer_rsrc_t rs =
+                    __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, 0x7fffffff, 0x00020000);
+                const bool live = FULL || (col_in && r0 + row + (int64_t)i * RSTEP < R);
+                r[i] = __builtin_bit_cast(vec_t, __builtin_amdgcn_raw_buffer_load_b128(rs, live ? off : kOut, 0, 0));
             }
             return;
         }
+        const bool cols_in = c0 + TC <= Cn;
+        sh = 0;
 #pragma unroll
         for (int i = 0; i < NCH; ++i) {
             const int64_t gr = r0 + row + (int64_t)i * RSTEP, gc = c0 + col;
@@ -182,23 +180,52 @@ struct Stager {
         }
     }
 
+    // Vector v loaded sh elements early: element e is v[e + sh], zero past
+    // the end.
+    static __device__ __forceinline__ vec_t unshift(vec_t v, int sh)
+    {
+        vec_t o;
+#pragma unroll
+        for (int e = 0; e < Cf::V; ++e) {
+            T x = v[e];
+#pragma unroll
+            for (int t = 1; t < Cf::V; ++t) x = (sh == t) ? (e + t < Cf::V ? v[e + t] : T(0)) : x;
+            o[e] = x;
+        }
+        return o;
+    }
+
+    // SHIFT: undo the load's shift of vectors that end past the last column.
+    template <bool SHIFT>
     __device__ __forceinline__ void store(T *s, int tid) const
     {
 #pragma unroll
         for (int i = 0; i < NCH; ++i) {
             const int c = tid + i * Cf::THREADS;
             const int row = c / CPR, col = (c % CPR) * Cf::V;
-            *reinterpret_cast<vec_t *>(s + row * S + col) = r[i];
+            *reinterpret_cast<vec_t *>(s + row * S + col) = SHIFT ? unshift(r[i], sh) : r[i];
         }
     }
 };
 
-// The FULL body's lane offsets (up to BM rows of a stored operand tile) stay
-// below 2^31 bytes for leading dimension ld.
+// The buffer loads' lane offsets (up to BM rows of a stored operand tile)
+// stay below 2^31 bytes for leading dimension ld.
 template <typename T, int BM>
 inline bool full_offsets_fit(int64_t ld)
 {
     return (double)(BM - 1) * (double)ld * sizeof(T) + 4096.0 < 2147483647.0;
+}
+
+// The tile body may take buffer loads (PTR = false) for g: both leading
+// dimensions within full_offsets_fit and both stored operands at least one
+// 16-byte vector wide (a vector that ends past the last column is loaded
+// shifted back inside the row).
+template <typename T, int BM>
+inline bool buffer_ok(const Gemm<T> &g)
+{
+    constexpr int64_t V = 16 / (int64_t)sizeof(T);
+    const int64_t ac = g.ta ? g.M : g.K, bc = g.tb ? g.K : g.N;
+    return full_offsets_fit<T, BM>(g.lda) && full_offsets_fit<T, BM>(g.ldb) && ac >= V && bc >= V;
 }
 
 // True when tile (m0, n0) over K range [kb, ke) can take the FULL body: the
@@ -231,8 +258,9 @@ constexpr bool store_prio()
 // One BM x BM tile of C at (m0, n0) with NW waves, over K range [kb, ke)
 // (kb a multiple of BK).  lds: Cfg::LDS_ELEMS elements.  Every wave of the
 // workgroup must call it (it contains barriers).  FULL (see full_tile): no
-// bounds checks anywhere.  PTR: 64-bit operand addressing (both leading
-// dimensions must pass full_offsets_fit otherwise).  W == nullptr: C = alpha
+// bounds checks anywhere.  PTR: 64-bit operand addressing (otherwise both
+// leading dimensions must pass full_offsets_fit and both stored operands have
+// at least V columns: buffer_ok).  W == nullptr: C = alpha
 // acc + beta C; otherwise the raw sums go to W (M x N, leading dimension N), a
 // split-K partial.  The MFMA sequence of every C element is the same for any
 // NW, FULL and PTR, so those variants give bit-identical results.
@@ -272,9 +300,10 @@ __device__ __forceinline__ void tile_body(const Gemm<T> &g, int64_t m0, int64_t 
         for (int j = 0; j < NJ; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
     const int nk = (int)(FULL ? (ke - kb) / BK : (ke - kb + BK - 1) / BK);
+    constexpr bool SHIFT = !FULL && !PTR;
     load_tiles(kb);
-    sa.store(stage(0, 0), tid);
-    sb.store(stage(0, 1), tid);
+    sa.template store<SHIFT>(stage(0, 0), tid);
+    sb.template store<SHIFT>(stage(0, 1), tid);
     __syncthreads();
 
     for (int kt = 0; kt < nk; ++kt) {
@@ -308,8 +337,8 @@ __device__ __forceinline__ void tile_body(const Gemm<T> &g, int64_t m0, int64_t 
         if constexpr (mfma_prio<T, BM, NW, TA, TB>()) __builtin_amdgcn_s_setprio(0);
         if constexpr (store_prio<T, BM, NW, TA, TB>()) __builtin_amdgcn_s_setprio(1);
         if (more) {
-            sa.store(stage(cur ^ 1, 0), tid);
-            sb.store(stage(cur ^ 1, 1), tid);
+            sa.template store<SHIFT>(stage(cur ^ 1, 0), tid);
+            sb.template store<SHIFT>(stage(cur ^ 1, 1), tid);
         }
         __syncthreads();
         if constexpr (store_prio<T, BM, NW, TA, TB>()) __builtin_amdgcn_s_setprio(0);
