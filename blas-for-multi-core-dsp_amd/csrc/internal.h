@@ -60,9 +60,14 @@ bool skinny_fills(const Gemm<T> &g, int num_cus);
 // hi/lo with 3 products.  `work` must hold split_workspace_bytes(...) bytes.
 size_t split_workspace_bytes(int64_t M, int64_t N, int64_t K, int scheme);
 hipError_t launch_split_gemm(const Gemm<float> &g, int scheme, void *work, hipStream_t s);
-// The six-product BF16 form on fragment-tiled planes (gemm_split_frag.hip);
-// `work` must hold split_workspace_bytes(M, N, K, 0) bytes.
-hipError_t launch_split6_frag(const Gemm<float> &g, void *work, hipStream_t s);
+// The split forms on fragment-tiled pieces (gemm_split_frag.hip): splits
+// op(A) into Ap (pieces x Mp x Kp) and op(B)^T into Bp (pieces x Np x Kp),
+// then runs the GEMM.  Mp a multiple of 256, Np of 128, Kp of 64.  The scaled
+// FP16 form (scheme 2) takes power-of-two row scales of op(A) and op(B)^T and
+// their inverses (Mp and Np entries).
+hipError_t launch_frag_gemm(const Gemm<float> &g, int scheme, uint16_t *Ap, uint16_t *Bp, int64_t Mp, int64_t Np,
+                            int64_t Kp, const float *a_scale, const float *a_inv, const float *b_scale,
+                            const float *b_inv, hipStream_t s);
 
 // FP32 GEMM on the FP8 (E4M3) matrix cores with per-128-block power-of-two
 // scales (gemm_fp8.hip).  `work` must hold fp8_workspace_bytes(...) bytes.
