@@ -12,15 +12,25 @@
 //       codes of x 2^-e (round to nearest even) K-contiguously, zero-padded to
 //       128-multiples in every dimension, plus the biased exponents 127 + e
 //       (E8M0 scale bytes);
-//   fp8_gemm_kernel  128 x 128 output tiles, K steps of 128 = one
+//   fp8_big_kernel  128 x 128 output tiles, K steps of 128 = one
 //       v_mfma_scale_f32_16x16x128_f8f6f4 depth, so each step's scales are
-//       one byte per row / column, applied by the matrix core itself.
+//       one byte per row / column, applied by the matrix core itself; the
+//       quantized operands go global -> LDS by direct-to-LDS loads into an
+//       XOR-swizzled layout, two workgroups per CU.  At 8192^3 on MI355X the
+//       whole form (quantization included) went from 0.88-0.90 ms with
+//       fp8_gemm_kernel (the same tiles staged through registers into
+//       padded LDS rows) to 0.74-0.76 ms, bit-identical; 256 x 128 and
+//       256 x 256 tiles at one workgroup per CU ran slower (0.97-1.0 ms).
+//       fp8_gemm_kernel remains for K past the direct loads' 32-bit lane
+//       offsets (8 Kp >= 2^31).
 //
 // MFMA layout, measured with a lane probe: A lane l holds 32 bytes of row
 // l % 16 at k = 32 (l / 16) .. +31 (B the same for column l % 16, the same k),
 // lane l's scale byte applies to row (column) l % 16 and k block l / 16, and
 // output register r of lane l is row 4 (l / 16) + r, column l % 16.
 #include <math.h>
+
+#include <type_traits>
 
 #include "api_common.h"
 
@@ -284,6 +294,158 @@ fp8_gemm_kernel(const uint8_t *__restrict__ Qa, const uint8_t *__restrict__ Qb, 
             }
 }
 
+// The direct-to-LDS GEMM, TM x TN tiles of 4 waves ((TM / 2) x (TN / 2)
+// each).  Per K step of 128 a workgroup needs TM + TN rows x 128 bytes; they
+// go global -> LDS by direct-to-LDS buffer loads (lane l's 16 bytes land at
+// the block base + 16 l), 1 KiB each, into two LDS stages, one step ahead.  A
+// 1 KiB block holds 8 rows; the 16-byte chunk c of row r sits in slot 8 r +
+// (c ^ (r & 7)), so lane l of a load reads row l / 8, chunk (l % 8) ^ (l / 8),
+// and a fragment read (16 rows, the same chunk) touches 8 distinct bank groups
+// per 8 lanes: no bank conflicts.
+template <int TM, int TN>
+struct BigTile {
+    static constexpr int BLOCKS_A = TM / 8, BLOCKS = (TM + TN) / 8;  // 1 KiB blocks per stage
+    static constexpr int LOADS = BLOCKS / 4;                         // per wave
+    static constexpr int MI = TM / 32, NJ = TN / 32;                 // MFMA tiles per wave
+    static constexpr int WPE = BLOCKS * 2 <= 80 ? 2 : 1;             // workgroups per CU the LDS allows
+};
+
+// s_waitcnt vmcnt(0) through the builtin, so the compiler's own wait
+// insertion sees it (gfx9 encoding; expcnt and lgkmcnt left at their maxima).
+__device__ __forceinline__ void wait_vmcnt0() { __builtin_amdgcn_s_waitcnt((7 << 4) | (15 << 8)); }
+
+// This file is compiled with the MFMAs in their VGPR form (see the
+// Makefile): otherwise, with the register budget of one wave per SIMD, the
+// compiler parks the accumulators in AGPRs and copies them around every MFMA.
+template <int TM, int TN>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BigTile<TM, TN>::WPE, BigTile<TM, TN>::WPE)))
+fp8_big_kernel(const uint8_t *__restrict__ Qa, const uint8_t *__restrict__ Qb, const uint32_t *__restrict__ Sa,
+               const uint32_t *__restrict__ Sb, int64_t Kp, int64_t Mp, int64_t Np, int tiles_m, int tiles_n,
+               float *__restrict__ C, int64_t ldc, int64_t M, int64_t N, float alpha, float beta)
+{
+    using Bt = BigTile<TM, TN>;
+    constexpr int MI = Bt::MI, NJ = Bt::NJ;
+    __shared__ __attribute__((aligned(1024))) uint8_t lds0[Bt::BLOCKS * 1024];
+    __shared__ __attribute__((aligned(1024))) uint8_t lds1[Bt::BLOCKS * 1024];
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lr = lane & 15, lg = lane >> 4;
+    const int wm = (wave >> 1) * (TM / 2), wn = (wave & 1) * (TN / 2);
+
+    // XCD-aware grouped tile order, as in the other GEMM kernels.
+    int pid = blockIdx.x;
+    const int nwg = gridDim.x;
+    if ((nwg & 7) == 0) pid = (pid & 7) * (nwg >> 3) + (pid >> 3);
+    const int width = 8 * tiles_n;
+    const int first_m = (pid / width) * 8;
+    const int gsz = min(tiles_m - first_m, 8);
+    const int tm = first_m + (pid % width) % gsz, tn = (pid % width) / gsz;
+    const int64_t m0 = (int64_t)tm * TM, n0 = (int64_t)tn * TN;
+
+    const int nk = (int)(Kp / kQB);
+    // lane l of a block load: row l / 8 of the block, chunk (l % 8) ^ (l / 8)
+    const uint32_t voff = (uint32_t)((lane >> 3) * Kp + (((lane & 7) ^ (lane >> 3)) * 16));
+    auto stage = [&](auto s_c) -> uint8_t * { return decltype(s_c)::value == 0 ? lds0 : lds1; };
+    const int64_t a_groups = Mp / kRowBlock, b_groups = Np / kRowBlock;
+    const int64_t a_grp = (m0 + wm) / kRowBlock, b_grp = (n0 + wn) / kRowBlock;
+    constexpr int GA = (MI + 3) / 4, GB = (NJ + 3) / 4;  // scale words (64-row groups) per operand
+    uint32_t sc[2][GA + GB];
+    auto issue = [&](int kb, auto s_c) {
+        constexpr int BB = decltype(s_c)::value;
+        const int64_t k = min(kb, nk - 1);
+        uint8_t *dst = stage(s_c);
+#pragma unroll
+        for (int q = 0; q < Bt::LOADS; ++q) {
+            const int b = wave * Bt::LOADS + q;  // wave-uniform
+            const bool isb = b >= Bt::BLOCKS_A;
+            const int rows = (isb ? b - Bt::BLOCKS_A : b) * 8;
+            const uint8_t *base = (isb ? Qb + (n0 + rows) * Kp : Qa + (m0 + rows) * Kp) + k * kQB;
+            const __amdgpu_buffer_rsrc_t rs =
+                __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t *>(base), (short)0, 0x7fffffff, 0x00020000);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void *)(dst + b * 1024),
+                                                     16, voff, 0, 0, 0);
+        }
+#pragma unroll
+        for (int g = 0; g < GA; ++g) sc[BB][g] = Sa[(k * a_groups + a_grp + g) * 16 + lr];
+#pragma unroll
+        for (int g = 0; g < GB; ++g) sc[BB][GA + g] = Sb[(k * b_groups + b_grp + g) * 16 + lr];
+    };
+
+    f4 acc[MI][NJ];
+#pragma unroll
+    for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) acc[i][j] = f4{0, 0, 0, 0};
+
+    // Step kt on buffer kt % 2: wait for this wave's loads of stage kt,
+    // barrier (every wave's have landed; every wave is done with the other
+    // buffer), load stage kt + 1 into the other buffer, read this wave's
+    // fragments, issue its MFMAs.  Past the last stage a step reloads the
+    // last one (the same wait every step); that buffer is never read.
+    auto step = [&](int kt, auto b_c) {
+        constexpr int BB = decltype(b_c)::value;
+        __builtin_amdgcn_sched_barrier(0);
+        wait_vmcnt0();
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_sched_barrier(0);
+        issue(kt + 1, std::integral_constant<int, 1 - BB>());
+        const uint8_t *base = stage(b_c);
+        auto frag = [&](const uint8_t *op, int row) -> i8v {
+            const int c0 = 2 * lg, sw = row & 7;
+            const u32x4 x0 = *reinterpret_cast<const u32x4 *>(op + row * 128 + ((c0 ^ sw) * 16));
+            const u32x4 x1 = *reinterpret_cast<const u32x4 *>(op + row * 128 + (((c0 + 1) ^ sw) * 16));
+            return i8v{(int)x0[0], (int)x0[1], (int)x0[2], (int)x0[3], (int)x1[0], (int)x1[1], (int)x1[2], (int)x1[3]};
+        };
+        // op(B)'s fragments stay in registers for the step; op(A)'s are read
+        // one row tile at a time, each used by NJ MFMAs.
+        i8v fb[NJ];
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) fb[j] = frag(base + TM * 128, wn + 16 * j + lr);
+#pragma unroll
+        for (int i = 0; i < MI; ++i) {
+            const i8v fa = frag(base, wm + 16 * i + lr);
+            const int sai = (int)((sc[BB][i >> 2] >> (8 * (i & 3))) & 0xFFu);
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) {
+                const int sbj = (int)((sc[BB][GA + (j >> 2)] >> (8 * (j & 3))) & 0xFFu);
+                acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(fa, fb[j], acc[i][j], 0, 0, 0, sai, 0,
+                                                                             sbj);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    using B0 = std::integral_constant<int, 0>;
+    using B1 = std::integral_constant<int, 1>;
+    issue(0, B0());
+    int kt = 0;
+    for (; kt + 2 <= nk; kt += 2) {
+        step(kt, B0());
+        step(kt + 1, B1());
+    }
+    if (kt < nk) step(kt, B0());
+    wait_vmcnt0();  // no load into this workgroup's LDS may be in flight when it exits
+
+#pragma unroll
+    for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int j = 0; j < NJ; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t row = m0 + wm + i * 16 + 4 * lg + r, col = n0 + wn + j * 16 + lr;
+                if (row < M && col < N) {
+                    float v = alpha * acc[i][j][r];
+                    if (beta != 0.0f) v += beta * C[row * ldc + col];
+                    C[row * ldc + col] = v;
+                }
+            }
+}
+
+// Tile of the direct-to-LDS FP8 GEMM.
+constexpr int kBigM = 128, kBigN = 128;
+
 struct Fp8Layout {
     int64_t Mp, Np, Kp, KB;
     uint8_t *Qa, *Qb, *Sa, *Sb;
@@ -295,8 +457,8 @@ size_t up256(size_t b) { return (b + 255) / 256 * 256; }
 Fp8Layout fp8_layout(int64_t M, int64_t N, int64_t K, void *base)
 {
     Fp8Layout L;
-    L.Mp = (M + kTile - 1) / kTile * kTile;
-    L.Np = (N + kTile - 1) / kTile * kTile;
+    L.Mp = (M + kBigM - 1) / kBigM * kBigM;
+    L.Np = (N + kBigN - 1) / kBigN * kBigN;
     L.KB = (K + kQB - 1) / kQB;
     L.Kp = L.KB * kQB;
     uint8_t *p = (uint8_t *)base;
@@ -343,6 +505,15 @@ hipError_t launch_fp8_gemm(const Gemm<float> &g, void *work, hipStream_t s)
     hipError_t e = quantize(g.A, g.lda, !g.ta, g.M, g.K, L.Mp, L.KB, L.Qa, L.Sa, s);
     if (e == hipSuccess) e = quantize(g.B, g.ldb, g.tb, g.N, g.K, L.Np, L.KB, L.Qb, L.Sb, s);
     if (e != hipSuccess) return e;
+    // the big kernel's lane offsets span 8 rows of the quantized operands
+    if (8.0 * (double)L.Kp + 128.0 < 2147483647.0) {
+        if ((L.Mp / kBigM) * (L.Np / kBigN) > 0x7fffffffLL) return hipErrorInvalidValue;
+        const int tiles_m = (int)(L.Mp / kBigM), tiles_n = (int)(L.Np / kBigN);
+        hipLaunchKernelGGL((fp8_big_kernel<kBigM, kBigN>), dim3((unsigned)(tiles_m * tiles_n)), dim3(256), 0, s, L.Qa, L.Qb,
+                           (const uint32_t *)L.Sa, (const uint32_t *)L.Sb, L.Kp, L.Mp, L.Np, tiles_m, tiles_n, g.C,
+                           g.ldc, g.M, g.N, g.alpha, g.beta);
+        return hipGetLastError();
+    }
     if ((L.Mp / kTile) * (L.Np / kTile) > 0x7fffffffLL) return hipErrorInvalidValue;
     const int tiles_m = (int)(L.Mp / kTile), tiles_n = (int)(L.Np / kTile);
     hipLaunchKernelGGL(fp8_gemm_kernel, dim3((unsigned)(tiles_m * tiles_n)), dim3(256), 0, s, L.Qa, L.Qb,
