@@ -265,6 +265,12 @@ def desc(transA: str, transB: str, A, B, C, alpha: float = 1.0, beta: float = 0.
                     B.data_ptr(), _ld(B), C.data_ptr(), _ld(C))
 
 
+def desc_array(descs):
+    """A ctypes GemmDesc array of `descs`, as blasdsp_gemm_grouped and
+    blasdsp_group_plan_run take it."""
+    return (GemmDesc * len(descs))(*descs)
+
+
 def gemm_grouped(descs, stream=None) -> None:
     """All problems of `descs` (GemmDesc list) in one grouped launch."""
     arr = (GemmDesc * len(descs))(*descs)
@@ -279,8 +285,10 @@ class GroupPlan:
         check("blasdsp_group_plan_create", lib().blasdsp_group_plan_create(max_problems, ctypes.byref(self._p)))
 
     def run(self, descs, stream=None) -> None:
-        arr = (GemmDesc * len(descs))(*descs)
-        check("blasdsp_group_plan_run", lib().blasdsp_group_plan_run(self._p, arr, len(descs), _stream(stream)))
+        """descs: a list of GemmDesc, or a ctypes GemmDesc array built once
+        (desc_array) and reused across runs."""
+        arr = descs if isinstance(descs, ctypes.Array) else desc_array(descs)
+        check("blasdsp_group_plan_run", lib().blasdsp_group_plan_run(self._p, arr, len(arr), _stream(stream)))
 
     def close(self) -> None:
         if self._p:
