@@ -18,16 +18,23 @@
 // four hardware queues, so they start together and share the device.
 //
 // The problem list lives in device memory and the grids do not depend on it,
-// so one instantiated hipGraph (built from explicit kernel nodes, no stream
-// capture and no memset nodes) replays any group: the host uploads the list,
-// with the tile counters at zero, and launches the graph.
+// so an instantiated hipGraph (built from explicit kernel nodes, no stream
+// capture and no memset nodes) replays any group that fills its lists: the
+// host uploads the list, with the tile counters at zero, and launches the
+// graph of the lists that have problems.
 //
 // On 512 requests of a mixed workload (M, N, K log-uniform in [16, 4096], 30 %
 // with one side 8-32, 70 % FP32, every mode, unaligned leading dimensions) on
 // MI355X: seven kernels (FP64 128 tiles one per mode) with a fixed stride
 // over the tiles took 5.8 ms, the three FP64 128 kernels starting only when
 // one of the first four had finished; four kernels with the tile counter 5.4
-// ms; loads at any alignment in the tile body 4.5 ms.
+// ms; loads at any alignment in the tile body 4.5 ms.  Tiles handed out in
+// list order go to whichever workgroup asks, so the tiles one XCD runs
+// together share few A and B panels (L2 hit rate 0.40 on one 4095 x 4096 x
+// 4096 FP64 problem, 0.74 for the per-call kernel's XCD-aware order).  Per-XCD
+// tile queues (blocks of 8 x 8 tiles dealt to the XCDs, workgroups stealing
+// from the other queues once their own is empty) raised it to 0.73 and ran no
+// faster: 4.59 against 4.50 ms on the mixed workload.
 #include <stdlib.h>
 
 #include <algorithm>
@@ -43,13 +50,13 @@ namespace {
 using tile::Cfg;
 
 // Waves per workgroup: 128 tiles take the 8-wave body (32 x 64 per wave) of
-// the per-call kernel, 64 tiles 4 waves (32 x 32 each).  With the group's
-// problem lookup and all four modes the 8-wave FP64 body spills 16
-// registers; on the mixed workload above, with one kernel per mode (4-25
-// spilled), it ran 3 % faster than 4 waves for FP64, and 8 waves for both
-// precisions 7 % faster than 4 for both.  With all modes in one kernel, 4
-// waves for FP64 (no spills, two workgroups per CU) ran uniform 2048^3
-// groups 6-30 % faster but the mixed workload's FP64 128 list 4 % slower.
+// the per-call FP64 kernel, 64 tiles 4 waves (32 x 32 each).  On the mixed
+// workload below, with one kernel per mode, 8 waves for FP64 ran 3 % faster
+// than 4, and 8 waves for both precisions 7 % faster than 4 for both.  With
+// all modes in one kernel: 4 waves for FP64 ran uniform 2048^3 groups 6-30 %
+// faster but the mixed workload's FP64 128 list 4 % slower; 4 waves for FP32
+// (the per-call FP32 kernel's 64 x 64 per wave) ran one 4095 x 4096 x 4096 NN
+// problem 16 % faster but the mixed workload's FP32 128 list 9 % slower.
 template <typename T, int BM>
 constexpr int group_nw()
 {
@@ -57,7 +64,7 @@ constexpr int group_nw()
 }
 
 template <typename T, int BM, bool TA, bool TB>
-__device__ __forceinline__ void run_tile(const GroupProblem &P, int64_t local, T *lds)
+__device__ __forceinline__ void run_tile(const GroupProblem &P, int64_t local, T *lds, int tid)
 {
     Gemm<T> g;
     g.ta = TA; g.tb = TB;
@@ -67,7 +74,7 @@ __device__ __forceinline__ void run_tile(const GroupProblem &P, int64_t local, T
     g.B = (const T *)P.B; g.ldb = P.ldb;
     g.C = (T *)P.C; g.ldc = P.ldc;
     const int64_t tm = local / P.tiles_n, tn = local % P.tiles_n;
-    tile::tile_body<T, BM, TA, TB, group_nw<T, BM>()>(g, tm * BM, tn * BM, lds);
+    tile::tile_body<T, BM, TA, TB, group_nw<T, BM>()>(g, tm * BM, tn * BM, lds, 0, -1, nullptr, tid);
 }
 
 template <typename T, int BM>
@@ -87,7 +94,12 @@ grouped_kernel(GroupHeader *__restrict__ hdr, const GroupProblem *__restrict__ p
     for (;;) {
         if (threadIdx.x == 0) *s_tile = atomicAdd(&hdr->next[w], 1ull);
         __syncthreads();
-        const int64_t t = (int64_t)*s_tile;
+        // Read through readfirstlane: the value is the same in every lane, and
+        // a scalar tile number keeps the problem lookup and the problem's
+        // fields in scalar registers.
+        const unsigned long long tv = *s_tile;
+        const int64_t t = (int64_t)(((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(tv >> 32)) << 32) |
+                                    (unsigned)__builtin_amdgcn_readfirstlane((unsigned)tv));
         __syncthreads();
         if (t >= total) break;  // the same value in every wave: all leave together
         int lo = 0, hi = count - 1;
@@ -98,11 +110,20 @@ grouped_kernel(GroupHeader *__restrict__ hdr, const GroupProblem *__restrict__ p
         }
         const GroupProblem &P = probs[lo];
         const int64_t local = t - P.tile_begin;
+        // FP64 128 tiles: an opaque copy of the thread index, so the
+        // per-thread offsets derived from it are computed inside each tile
+        // instead of being kept live across the loop.  Kept live they spilled
+        // (80 bytes per lane), and the reloads inside the K loop waited on the
+        // tile's global loads: 2.10 -> 1.91 ms for the mixed workload's FP64
+        // 128 list, 2.85 -> 2.51 ms for eight 2048^3 FP64 GEMMs.  The other
+        // three kernels do not spill and ran 5-18 % slower with it.
+        int tid = threadIdx.x;
+        if constexpr (sizeof(T) == 8 && BM == 128) asm volatile("" : "+v"(tid));
         switch ((P.ta << 1) | P.tb) {
-        case 0: run_tile<T, BM, false, false>(P, local, lds); break;
-        case 1: run_tile<T, BM, false, true>(P, local, lds); break;
-        case 2: run_tile<T, BM, true, false>(P, local, lds); break;
-        default: run_tile<T, BM, true, true>(P, local, lds); break;
+        case 0: run_tile<T, BM, false, false>(P, local, lds, tid); break;
+        case 1: run_tile<T, BM, false, true>(P, local, lds, tid); break;
+        case 2: run_tile<T, BM, true, false>(P, local, lds, tid); break;
+        default: run_tile<T, BM, true, true>(P, local, lds, tid); break;
         }
         __syncthreads();  // the next tile number and stages reuse the LDS
     }

@@ -263,17 +263,18 @@ constexpr bool store_prio()
 // at least V columns: buffer_ok).  W == nullptr: C = alpha
 // acc + beta C; otherwise the raw sums go to W (M x N, leading dimension N), a
 // split-K partial.  The MFMA sequence of every C element is the same for any
-// NW, FULL and PTR, so those variants give bit-identical results.
+// NW, FULL and PTR, so those variants give bit-identical results.  tid: the
+// thread's index in the workgroup (threadIdx.x when negative).
 template <typename T, int BM, bool TA, bool TB, int NW = 4, bool FULL = false, bool PTR = false>
 __device__ __forceinline__ void tile_body(const Gemm<T> &g, int64_t m0, int64_t n0, T *lds, int64_t kb = 0,
-                                          int64_t ke = -1, T *W = nullptr)
+                                          int64_t ke = -1, T *W = nullptr, int tid = -1)
 {
     if (ke < 0) ke = g.K;
+    if (tid < 0) tid = threadIdx.x;
     using Cf = Cfg<T, BM, NW>;
     using acc_t = typename Mma<T>::acc_t;
     constexpr int BK = Cf::BK, SK = Cf::SK, SMN = Cf::SMN, MI = Cf::MI, NJ = Cf::NJ, BUF = Cf::BUF;
 
-    const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int lr = lane & 15, lk = lane >> 4;
     const int wm = (wave / Cf::WGN) * Cf::WM, wn = (wave % Cf::WGN) * Cf::WN;

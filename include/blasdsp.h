@@ -319,10 +319,12 @@ typedef struct blasdsp_gemm_desc {
 /* Runs the whole group asynchronously on `stream` (a cached plan per stream). */
 BLASDSP_API int blasdsp_gemm_grouped(const blasdsp_gemm_desc *descs, int count, void *stream);
 
-/* A reusable plan: device problem list, pinned staging and an instantiated
- * hipGraph of the grouped kernels.  run() uploads the list and replays the
- * graph on `stream`; a run on another stream than the previous one waits for
- * the previous replay first.  A plan is used by one host thread at a time. */
+/* A reusable plan: device problem list, pinned staging and instantiated
+ * hipGraphs of the grouped kernels (one per set of problem lists a group can
+ * fill).  run() uploads the list and replays, on `stream`, the graph of the
+ * kernels the group needs; a run on another stream than the previous one
+ * waits for the previous replay first.  A plan is used by one host thread at
+ * a time. */
 typedef struct blasdsp_group_plan blasdsp_group_plan;
 BLASDSP_API int blasdsp_group_plan_create(int max_problems, blasdsp_group_plan **plan);
 BLASDSP_API int blasdsp_group_plan_run(blasdsp_group_plan *plan, const blasdsp_gemm_desc *descs,

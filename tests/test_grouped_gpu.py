@@ -126,6 +126,48 @@ def test_group_plan_two_streams_and_large_tiles(orc, blas, dev):
         plan.close()
 
 
+def test_group_plan_list_subsets(orc, blas, dev):
+    """One plan replaying, back to back on one stream, groups that fill
+    different sets of the four problem lists (FP32 only, FP64 only, 64 tiles
+    only, 128 tiles only, one list each, all four), so each replay runs a
+    different graph; every problem bit for bit against the problem alone."""
+    import torch
+
+    def problem(prec, big, seed):
+        dt = np.float64 if prec == "d" else np.float32
+        rng = np.random.default_rng(seed)
+        mode = "NT"[rng.integers(2)] + "NT"[rng.integers(2)]
+        M, N = (int(x) for x in rng.integers(384, 520, 2)) if big else (int(x) for x in rng.integers(5, 60, 2))
+        K = int(rng.integers(1, 300))
+        A = orc.matrix(M, K, 0, seed, dtype=dt, trans=mode[0] == "T")
+        B = orc.matrix(K, N, 0, seed + 1, dtype=dt, trans=mode[1] == "T")
+        C0 = orc.matrix(M, N, 0, seed + 2, dtype=dt)
+        return dict(mode=mode, A=to_dev(A, dev), B=to_dev(B, dev), C0=to_dev(C0, dev))
+
+    kinds = {"s64": ("s", False), "s128": ("s", True), "d64": ("d", False), "d128": ("d", True)}
+    groups = [("s64", "s128"), ("d64", "d128"), ("s64", "d64"), ("s128", "d128"), ("d128",), ("s64",),
+              ("s128", "d64"), ("s64", "s128", "d64", "d128")]
+    plan = blas.GroupPlan(16)
+    try:
+        runs = []
+        for g, kset in enumerate(groups):
+            probs = [problem(*kinds[k], 1000 * g + 10 * i) for i, k in enumerate(kset * 2)]
+            outs = [p["C0"].clone() for p in probs]
+            plan.run([blas.desc(p["mode"][0], p["mode"][1], p["A"], p["B"], c, -0.5, 1.5)
+                      for p, c in zip(probs, outs)])
+            runs.append((probs, outs))
+        torch.cuda.synchronize()
+        for g, (probs, outs) in enumerate(runs):
+            for i, (p, c) in enumerate(zip(probs, outs)):
+                alone = p["C0"].clone()
+                blas.gemm(p["mode"][0], p["mode"][1], p["A"], p["B"], alone, -0.5, 1.5, algo=blas.ALGO_TILE,
+                          splitk=1)
+                torch.cuda.synchronize()
+                assert torch.equal(c, alone), (groups[g], i)
+    finally:
+        plan.close()
+
+
 def test_group_wide_leading_dimension(orc, blas, dev):
     """A group holding an FP32 problem whose A rows are 4.4 million elements
     apart (past the grouped kernel's 32-bit load offsets: it runs through the
