@@ -23,11 +23,16 @@
 //
 // GEMM kernel.  A 256 x 128 tile of C per workgroup, 4 waves of 128 x 64
 // (chunk and running accumulators, 256 registers, in AGPRs).  Per 32-deep K
-// step a workgroup needs 72 fragments (op(A): 16 row blocks x 3 pieces,
-// op(B): 8 x 3); they go global -> LDS by direct-to-LDS buffer loads
-// (buffer_load_dwordx4 ... lds: lane l's 16 bytes land at the LDS base +
-// 16 l), no staging registers, into two LDS stages (144 KiB), one step
-// ahead.  Each step a wave reads its 36 fragments and issues 192 MFMAs.
+// step a workgroup needs 72 fragments with three pieces (op(A): 16 row
+// blocks x 3 pieces, op(B): 8 x 3), 48 with two; they go global -> LDS by
+// direct-to-LDS buffer loads (buffer_load_dwordx4 ... lds: lane l's 16
+// bytes land at the LDS base + 16 l), no staging registers, into a ring of
+// LDS stages (two of 72 KiB or three of 48 KiB).  Each step a wave reads
+// its 36 (24) fragments and issues 192 (96) MFMAs' worth of products.
+// What bounds it, measured at 8192^3 with timing-only copies (six products,
+// constant operands): the global -> LDS loads alone take 3.2 ms, the MFMAs
+// alone with the loads 3.3 ms, the whole kernel 5.0; the loads hit L2 79 %
+// of the time, the compulsory rate of the 8 x 4 tiles an XCD runs at once.
 // Measured at 8192^3 on MI355X (each change bit-identical in the tests):
 // row-major pieces with register staging and 128 x 128 tiles 8.46 ms per
 // call; fragment tiling, direct-to-LDS loads and a three-stage ring 6.96;
@@ -43,6 +48,7 @@ namespace {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
+typedef float f16v __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -52,34 +58,97 @@ constexpr int kTileRB = 16;     // op(A) row blocks of a 256-row tile
 constexpr int kChunkSteps = 6;  // 32-deep K steps per accumulation chunk (192)
 
 // The forms (scheme ids of gemm_split.hip).  NP pieces per operand; products
-// (piece of op(A) a[t], piece of op(B) b[t]), smallest first; NREAD: the
-// products whose pieces are read first-use first (all pieces are read by
-// then).  CHUNK: two-level accumulation (each 192-deep chunk from zero into
-// `part`, then added into `acc`).  FP16: scaled FP16 pieces, hi*hi into acc
-// and the cross terms (carrying 2^11) into part.
+// (piece of op(A) a[t], piece of op(B) b[t]) in issue order.  CHUNK:
+// two-level accumulation (each 192-deep chunk from zero into `part`, then
+// added into `acc`).  FP16: scaled FP16 pieces, hi*hi into acc and the cross
+// terms (carrying 2^11) into part.
+//
+// Fragment schedule of one K step (the pieces' registers are reused across
+// steps): the HEAD pieces (op(A) piece when hb = 0, op(B) when 1) are read
+// from LDS at the step's start; the others were read during the previous
+// step, each right after that step's last product using the register it
+// goes into (pf[t]: the piece read after product t, -1 none; pb[t]: op(B)).
+// The first product of a step uses only such prefetched pieces, so its
+// MFMAs start with no wait on LDS; the workgroup barrier that makes the next
+// stage readable sits after product BAR, before the first prefetch.
+// NSTAGE: LDS stages in the ring (72 KiB each with three pieces, 48 KiB
+// with two).  MF32: the products run on 32 x 32 x 16 MFMAs instead of
+// 16 x 16 x 32 ones (same fragments, other lane addresses).  Measured at
+// 8192^3 on MI355X, all versions in one process on the same pieces of
+// random operands (medians of 5): six BF16 products 5.32 -> 5.01 ms with this
+// schedule (5.12 with 32 x 32 MFMAs); three BF16 products 2.92 -> 2.72 (a
+// third stage and this schedule; 2.76 with 32 x 32); scaled FP16 2.86 ->
+// 2.80, and 2.71 with 32 x 32.  Four tile rows per XCD group in place of
+// eight (fewer bytes per step from beyond L2) measured within noise.
 template <int F>
 struct FragForm;
 template <>
 struct FragForm<0> {  // BF16, 6 products
-    static constexpr int NP = 3, NT = 6, NREAD = 3;
+    static constexpr int NP = 3, NT = 6, NSTAGE = 2;
+    static constexpr bool MF32 = false;
     static constexpr bool FP16 = false, CHUNK = true;
-    static constexpr int a[6] = {1, 0, 2, 0, 1, 0};  // mm, hl, lh, hm, mh, hh
-    static constexpr int b[6] = {1, 2, 0, 1, 0, 0};
+    static constexpr int a[6] = {1, 1, 2, 0, 0, 0};  // mm, mh, lh, hm, hl, hh
+    static constexpr int b[6] = {1, 0, 0, 1, 2, 0};
+    static constexpr int NH = 2, hb[2] = {1, 0}, hp[2] = {0, 0};  // op(B) hi, op(A) hi
+    static constexpr int BAR = 1;
+    static constexpr int pf[6] = {-1, 1, 2, 1, 2, -1};  // A mid, A lo, B mid, B lo
+    static constexpr int pb[6] = {0, 0, 0, 1, 1, 0};
 };
 template <>
 struct FragForm<1> {  // BF16, 3 products
-    static constexpr int NP = 2, NT = 3, NREAD = 2;
+    static constexpr int NP = 2, NT = 3, NSTAGE = 3;
+    static constexpr bool MF32 = false;
     static constexpr bool FP16 = false, CHUNK = true;
-    static constexpr int a[3] = {0, 1, 0};  // hl, lh, hh
-    static constexpr int b[3] = {1, 0, 0};
+    static constexpr int a[3] = {1, 0, 0};  // lh, hh, hl
+    static constexpr int b[3] = {0, 0, 1};
+    static constexpr int NH = 2, hb[2] = {0, 1}, hp[2] = {0, 1};  // op(A) hi, op(B) lo
+    static constexpr int BAR = 0;
+    static constexpr int pf[3] = {1, 0, -1};  // A lo, B hi
+    static constexpr int pb[3] = {0, 1, 0};
 };
 template <>
 struct FragForm<2> {  // scaled FP16, 3 products
-    static constexpr int NP = 2, NT = 3, NREAD = 2;
+    static constexpr int NP = 2, NT = 3, NSTAGE = 3;
+    static constexpr bool MF32 = true;
     static constexpr bool FP16 = true, CHUNK = false;
-    static constexpr int a[3] = {0, 1, 0};  // hl, lh (part), hh (acc)
-    static constexpr int b[3] = {1, 0, 0};
+    static constexpr int a[3] = {1, 0, 0};  // lh (part), hh (acc), hl (part)
+    static constexpr int b[3] = {0, 0, 1};
+    static constexpr int NH = 2, hb[2] = {0, 1}, hp[2] = {0, 1};
+    static constexpr int BAR = 0;
+    static constexpr int pf[3] = {1, 0, -1};
+    static constexpr int pb[3] = {0, 1, 0};
 };
+
+// Each piece of each operand is read exactly once per step: at the head or
+// as one product's prefetch, and that only after its last use in the step.
+template <int F>
+constexpr bool schedule_ok()
+{
+    using Fo = FragForm<F>;
+    for (int side = 0; side < 2; ++side)
+        for (int p = 0; p < Fo::NP; ++p) {
+            int reads = 0, at = -1;  // at: product after which it is read (-1: head)
+            for (int h = 0; h < Fo::NH; ++h)
+                if (Fo::hb[h] == side && Fo::hp[h] == p) ++reads;
+            for (int t = 0; t < Fo::NT; ++t)
+                if (Fo::pf[t] == p && Fo::pb[t] == side) {
+                    ++reads;
+                    at = t;
+                    if (t < Fo::BAR) return false;  // before the barrier
+                }
+            if (reads != 1) return false;
+            for (int t = at + 1; t < Fo::NT && at >= 0; ++t)
+                if ((side == 0 ? Fo::a[t] : Fo::b[t]) == p) return false;  // used after its prefetch
+            // a head piece is never used by product 0 (its read is the step's first)
+            if (at < 0 && (side == 0 ? Fo::a[0] : Fo::b[0]) == p) return false;
+        }
+    return true;
+}
+static_assert(schedule_ok<0>() && schedule_ok<1>() && schedule_ok<2>(), "fragment schedule");
+
+// Buffer of step J of a chunk with NS stages.
+template <int J, int NS>
+using BufOf = std::integral_constant<int, J % NS>;
 
 template <int F>
 struct Stage {
@@ -205,6 +274,9 @@ __global__ void __launch_bounds__(256) split_frag_kernel(const float *__restrict
     }
 }
 
+// 16 x 16 x 32 product: lane l supplies row (column) l % 16 and k = 8 (l /
+// 16) .. + 7 of each operand (one fragment record); register r of lane l
+// holds output row 4 (l / 16) + r, column l % 16.
 template <bool FP16>
 __device__ __forceinline__ f4 mfma16(u32x4 a, u32x4 b, f4 c)
 {
@@ -213,6 +285,20 @@ __device__ __forceinline__ f4 mfma16(u32x4 a, u32x4 b, f4 c)
                                                       c, 0, 0, 0);
     else
         return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b),
+                                                       c, 0, 0, 0);
+}
+
+// 32 x 32 x 16 product: lane l supplies row (column) l % 32 and k = 8 (l /
+// 32) .. + 7 of each operand; register r of lane l holds output row 8 (r /
+// 4) + 4 (l / 32) + r % 4, column l % 32.
+template <bool FP16>
+__device__ __forceinline__ f16v mfma32(u32x4 a, u32x4 b, f16v c)
+{
+    if constexpr (FP16)
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b),
+                                                      c, 0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b),
                                                        c, 0, 0, 0);
 }
 
@@ -245,12 +331,12 @@ frag_kernel(const uint16_t *__restrict__ Ap, const uint16_t *__restrict__ Bp, in
     using St = Stage<F>;
     constexpr int NP = Fo::NP;
     constexpr int kStageA = St::A, kStageB = St::B, kLoadsPerWave = St::LOADS;
-    __shared__ __attribute__((aligned(1024))) uint16_t lds0[St::FRAGS * kFrag];
-    __shared__ __attribute__((aligned(1024))) uint16_t lds1[St::FRAGS * kFrag];
+    constexpr int NS = Fo::NSTAGE;
+    static_assert(NS * St::FRAGS * kFrag * 2 <= 160 * 1024, "stages fit in LDS");
+    __shared__ __attribute__((aligned(1024))) uint16_t lds[NS][St::FRAGS * kFrag];
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: scalar addressing
-    const int lr = lane & 15, lg = lane >> 4;
 
     // XCD-aware grouped tile order (as the tile kernel's).
     int pid = blockIdx.x;
@@ -271,7 +357,7 @@ frag_kernel(const uint16_t *__restrict__ Ap, const uint16_t *__restrict__ Bp, in
     const int64_t stepA = a_rb * NP * kFrag, stepB = b_rb * NP * kFrag;  // elements from one K block to the next
     const int voff = lane * 16;
     const int nk = (int)KB;  // 32-deep K steps
-    auto stage = [&](auto s_c) -> uint16_t * { return decltype(s_c)::value == 0 ? lds0 : lds1; };
+    auto stage = [&](auto s_c) -> uint16_t * { return lds[decltype(s_c)::value]; };
     auto issue = [&](int kb, auto s_c) {
         uint16_t *base = stage(s_c);
         const int64_t k = min(kb, nk - 1);
@@ -289,110 +375,163 @@ frag_kernel(const uint16_t *__restrict__ Ap, const uint16_t *__restrict__ Bp, in
         }
     };
 
-    // A wave owns 128 x 64 of the 256 x 128 tile: op(A) row blocks
-    // 8 (wave / 2) .. + 7, op(B) blocks 4 (wave % 2) .. + 3.
-    f4 acc[8][4], part[8][4];
+    // A wave owns 128 x 64 of the 256 x 128 tile: op(A) rows 128 (wave / 2)
+    // .., op(B) columns 64 (wave % 2) .., as TI x TJ MFMA blocks (16 x 16
+    // blocks, or 32 x 32 with MF32).
+    constexpr bool MF32 = Fo::MF32;
+    constexpr int TI = MF32 ? 4 : 8, TJ = MF32 ? 2 : 4, RR = MF32 ? 16 : 4;
+    using acc_t = typename std::conditional<MF32, f16v, f4>::type;
+    acc_t acc[TI][TJ], part[TI][TJ];
 #pragma unroll
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < TI; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            acc[i][j] = f4{0, 0, 0, 0};
-            part[i][j] = f4{0, 0, 0, 0};
-        }
+        for (int j = 0; j < TJ; ++j)
+#pragma unroll
+            for (int r = 0; r < RR; ++r) {
+                acc[i][j][r] = 0.0f;
+                part[i][j][r] = 0.0f;
+            }
 
-    // Step kt on buffer B = kt % 2: wait for this wave's loads of stage kt,
-    // barrier (every wave's loads of stage kt have landed; every wave is done
-    // with the other buffer), load stage kt + 1 into the other buffer, read
-    // this wave's 12 NP fragments and issue its 32 NT MFMAs.  Past the last
-    // stage a step reloads the last one (the same wait every step); that
-    // buffer is never read.
+    // This wave's fragments of each piece (8 of op(A), 4 of op(B)); they
+    // persist across steps.  16 x 16 blocks: fa[p][i] is 16-row fragment i
+    // as stored (lane l reads record l).  32 x 32 blocks: fa[p][2 i + h]
+    // (fb[p][2 j + h]) is the 32 x 16 operand of block i (j) for k = 16 h ..
+    // 16 h + 15 of the step: lane l's 16 bytes (row l % 32, k = 16 h + 8 (l /
+    // 32) .. + 7) are record (l % 16) + 16 (2 h + l / 32) of the 16-row
+    // fragment holding that row, so each 8-lane group still reads 128
+    // contiguous bytes.
+    u32x4 fa[NP][8], fb[NP][4];
+    const int lrec = MF32 ? (((lane & 31) >> 4) * NP * kFrag) + ((lane & 15) + 16 * (lane >> 5)) * 8 : lane * 8;
+    auto read_piece = [&](const uint16_t *base, int side, int p) {
+        if (side == 0) {
+#pragma unroll
+            for (int f = 0; f < 8; ++f) {
+                const int rb = MF32 ? 2 * (f >> 1) : f, ko = MF32 ? (f & 1) * 256 : 0;
+                fa[p][f] = *reinterpret_cast<const u32x4 *>(base + (((wave >> 1) * 8 + rb) * NP + p) * kFrag + lrec + ko);
+            }
+        } else {
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                const int cb = MF32 ? 2 * (f >> 1) : f, ko = MF32 ? (f & 1) * 256 : 0;
+                fb[p][f] = *reinterpret_cast<const u32x4 *>(base + (kStageA + ((wave & 1) * 4 + cb) * NP + p) * kFrag +
+                                                            lrec + ko);
+            }
+        }
+    };
+    auto product = [&](int t) {
+        const int pa = Fo::a[t], pb = Fo::b[t];
+        const bool to_acc = Fo::FP16 && pa == 0 && pb == 0;
+        if constexpr (MF32) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int i = 0; i < TI; ++i)
+#pragma unroll
+                    for (int j = 0; j < TJ; ++j) {
+                        if (to_acc) acc[i][j] = mfma32<Fo::FP16>(fa[pa][2 * i + h], fb[pb][2 * j + h], acc[i][j]);
+                        else part[i][j] = mfma32<Fo::FP16>(fa[pa][2 * i + h], fb[pb][2 * j + h], part[i][j]);
+                    }
+        } else {
+#pragma unroll
+            for (int i = 0; i < TI; ++i)
+#pragma unroll
+                for (int j = 0; j < TJ; ++j) {
+                    if (to_acc) acc[i][j] = mfma16<Fo::FP16>(fa[pa][i], fb[pb][j], acc[i][j]);
+                    else part[i][j] = mfma16<Fo::FP16>(fa[pa][i], fb[pb][j], part[i][j]);
+                }
+        }
+    };
+    constexpr int kMfmaPerProduct = TI * TJ * (MF32 ? 2 : 1);
+    // Step kt on buffer BB = kt % 2 (the prefetched pieces of stage kt are in
+    // registers): read the head pieces, issue products 0 .. BAR; wait for
+    // this wave's loads of stage kt + 1 and its reads of stage kt, barrier
+    // (stage kt + 1 has landed everywhere and no wave reads buffer BB any
+    // more), load stage kt + 2 into buffer BB, then the remaining products,
+    // each followed by the prefetch from stage kt + 1 of the piece it freed.
+    // Past the last stage a step reloads the last one (the same wait every
+    // step); those pieces are never used.
     auto step = [&](int kt, auto b_c) {
         constexpr int BB = decltype(b_c)::value;
+        const uint16_t *cur = stage(b_c);
+        const uint16_t *nxt = stage(std::integral_constant<int, (BB + 1) % NS>());
         __builtin_amdgcn_sched_barrier(0);
-        wait_vm_lgkm<0, 0>();
+#pragma unroll
+        for (int h = 0; h < Fo::NH; ++h) read_piece(cur, Fo::hb[h], Fo::hp[h]);
+#pragma unroll
+        for (int t = 0; t <= Fo::BAR; ++t) product(t);
+        __builtin_amdgcn_sched_group_barrier(0x100, Fo::NH == 2 ? 12 : 0, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, kMfmaPerProduct * (Fo::BAR + 1), 0);
+        __builtin_amdgcn_sched_barrier(0);
+        wait_vm_lgkm<(NS - 2) * kLoadsPerWave, 0>();  // stage kt + 1 (later stages may be in flight)
         barrier();
         __builtin_amdgcn_sched_barrier(0);
-        issue(kt + 1, std::integral_constant<int, 1 - BB>());
-        const uint16_t *base = stage(b_c);
-        u32x4 fa[NP][8], fb[NP][4];
-        // Fragments in the order the products first use them (six products:
-        // mid A and B, then hi A and lo B, then lo A and hi B; three: hi A and
-        // lo B, then lo A and hi B).
+        issue(kt + NS, b_c);
 #pragma unroll
-        for (int t = 0; t < Fo::NREAD; ++t) {
-            const int pa = Fo::a[t], pb = Fo::b[t];
-#pragma unroll
-            for (int i = 0; i < 8; ++i)
-                fa[pa][i] =
-                    *reinterpret_cast<const u32x4 *>(base + (((wave >> 1) * 8 + i) * NP + pa) * kFrag + lane * 8);
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                fb[pb][j] = *reinterpret_cast<const u32x4 *>(base + (kStageA + ((wave & 1) * 4 + j) * NP + pb) * kFrag +
-                                                             lane * 8);
+        for (int t = Fo::BAR + 1; t < Fo::NT; ++t) {
+            if (Fo::pf[t - 1] >= 0) read_piece(nxt, Fo::pb[t - 1], Fo::pf[t - 1]);
+            product(t);
         }
-#pragma unroll
-        for (int t = 0; t < Fo::NT; ++t)
-#pragma unroll
-            for (int i = 0; i < 8; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    if (!Fo::FP16 || t < Fo::NT - 1)
-                        part[i][j] = mfma16<Fo::FP16>(fa[Fo::a[t]][i], fb[Fo::b[t]][j], part[i][j]);
-                    else
-                        acc[i][j] = mfma16<Fo::FP16>(fa[Fo::a[t]][i], fb[Fo::b[t]][j], acc[i][j]);
-                }
-        // Order within the step: the stage loads; six products: the first
-        // two products' 24 LDS reads, the first product's 32 MFMAs (waiting
-        // for its own 12 reads), the third product's 12 reads, the other 160
-        // MFMAs; three products: the first product's 12 reads, its 32 MFMAs,
-        // the second's 12 reads, the other 64 MFMAs.
+        if (Fo::pf[Fo::NT - 1] >= 0) read_piece(nxt, Fo::pb[Fo::NT - 1], Fo::pf[Fo::NT - 1]);
+        // the stage loads, then per product its prefetched piece's reads
+        // (from the product before it) and its 32 MFMAs
         __builtin_amdgcn_sched_group_barrier(0x020, kLoadsPerWave, 0);
-        if constexpr (Fo::NT == 6) {
-            __builtin_amdgcn_sched_group_barrier(0x100, 24, 0);
-            __builtin_amdgcn_sched_group_barrier(0x008, 32, 0);
-            __builtin_amdgcn_sched_group_barrier(0x100, 12, 0);
-            __builtin_amdgcn_sched_group_barrier(0x008, 160, 0);
-        } else {
-            __builtin_amdgcn_sched_group_barrier(0x100, 12, 0);
-            __builtin_amdgcn_sched_group_barrier(0x008, 32, 0);
-            __builtin_amdgcn_sched_group_barrier(0x100, 12, 0);
-            __builtin_amdgcn_sched_group_barrier(0x008, 32 * (Fo::NT - 1), 0);
+#pragma unroll
+        for (int t = Fo::BAR + 1; t < Fo::NT; ++t) {
+            if (Fo::pf[t - 1] >= 0) {
+                if (Fo::pb[t - 1]) __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+                else __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
+            }
+            __builtin_amdgcn_sched_group_barrier(0x008, kMfmaPerProduct, 0);
         }
         __builtin_amdgcn_sched_barrier(0);
     };
     auto flush = [&]() {
         if constexpr (!Fo::CHUNK) return;
 #pragma unroll
-        for (int i = 0; i < 8; ++i)
+        for (int i = 0; i < TI; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                acc[i][j] += part[i][j];
-                part[i][j] = f4{0, 0, 0, 0};
-            }
+            for (int j = 0; j < TJ; ++j)
+#pragma unroll
+                for (int r = 0; r < RR; ++r) {
+                    acc[i][j][r] += part[i][j][r];
+                    part[i][j][r] = 0.0f;
+                }
     };
+    // step j of a chunk (chunks start at multiples of 6 steps, a multiple
+    // of the stage count) uses buffer j % NS
+    static_assert(kChunkSteps % NS == 0, "chunks start on buffer 0");
     using B0 = std::integral_constant<int, 0>;
     using B1 = std::integral_constant<int, 1>;
 
+    // Stages 0 .. NS - 1 issued, stage 0 landed everywhere, and step 0's
+    // prefetched pieces read from it.
     issue(0, B0());
+    if constexpr (NS == 3) issue(1, B1());
+    wait_vm_lgkm<(NS - 2) * kLoadsPerWave, 0>();
+    barrier();
+    issue(NS - 1, std::integral_constant<int, NS - 1>());
+#pragma unroll
+    for (int t = 0; t < Fo::NT; ++t)
+        if (Fo::pf[t] >= 0) read_piece(lds[0], Fo::pb[t], Fo::pf[t]);
     // Whole chunks of 6 steps (192-deep K) run unrolled, then the chunk is
     // added into the running sum; the last, partial chunk runs the same code
     // with each step guarded.
     int kt = 0;
     for (; kt + kChunkSteps <= nk; kt += kChunkSteps) {
-        step(kt + 0, B0());
-        step(kt + 1, B1());
-        step(kt + 2, B0());
-        step(kt + 3, B1());
-        step(kt + 4, B0());
-        step(kt + 5, B1());
+        step(kt + 0, BufOf<0, NS>());
+        step(kt + 1, BufOf<1, NS>());
+        step(kt + 2, BufOf<2, NS>());
+        step(kt + 3, BufOf<3, NS>());
+        step(kt + 4, BufOf<4, NS>());
+        step(kt + 5, BufOf<5, NS>());
         flush();
     }
     if (kt < nk) {
-        step(kt + 0, B0());
-        if (kt + 1 < nk) step(kt + 1, B1());
-        if (kt + 2 < nk) step(kt + 2, B0());
-        if (kt + 3 < nk) step(kt + 3, B1());
-        if (kt + 4 < nk) step(kt + 4, B0());
+        step(kt + 0, BufOf<0, NS>());
+        if (kt + 1 < nk) step(kt + 1, BufOf<1, NS>());
+        if (kt + 2 < nk) step(kt + 2, BufOf<2, NS>());
+        if (kt + 3 < nk) step(kt + 3, BufOf<3, NS>());
+        if (kt + 4 < nk) step(kt + 4, BufOf<4, NS>());
         flush();
     }
     // no load into this workgroup's LDS may be in flight when it exits
@@ -400,13 +539,14 @@ frag_kernel(const uint16_t *__restrict__ Ap, const uint16_t *__restrict__ Bp, in
 
     const int64_t m0 = (int64_t)tm * 256 + (wave >> 1) * 128, n0 = (int64_t)tn * 128 + (wave & 1) * 64;
 #pragma unroll
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < TI; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int64_t col = n0 + j * 16 + lr;
+        for (int j = 0; j < TJ; ++j) {
+            const int64_t col = MF32 ? n0 + j * 32 + (lane & 31) : n0 + j * 16 + (lane & 15);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int64_t row = m0 + i * 16 + 4 * lg + r;
+            for (int r = 0; r < RR; ++r) {
+                const int64_t row = MF32 ? m0 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)
+                                         : m0 + i * 16 + 4 * (lane >> 4) + r;
                 if (row < M && col < N) {
                     float *p = C + row * ldc + col;
                     float v;
