@@ -22,7 +22,8 @@
 //   * two LDS stages: global loads for K tile k+1 are in flight while the
 //     MFMAs consume tile k; one barrier per K tile;
 //   * epilogue C = alpha * acc + beta * C fused into the store (C is not read
-//     when beta == 0);
+//     when beta == 0; for a short K range C's lines are prefetched at the
+//     tile's start);
 //   * operand loads are buffer loads at any element alignment of the leading
 //     dimension; out-of-range rows and the ragged edge read as zero through
 //     the buffer's range check (see Stager::load);
