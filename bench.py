@@ -193,7 +193,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--workload", default="dgemm", choices=["dgemm", "skinny", "precision", "serving", "copy", "grouped",
+    ap.add_argument("--workload", default="dgemm", choices=["dgemm", "skinny", "precision", "serving", "copy", "mfma", "grouped",
                                                         "transpose", "sgemm512", "sweep", "stencil", "tune",
                                                         "batch"])
     args = ap.parse_args()
@@ -202,7 +202,7 @@ def main():
         sys.exit(2)
     if args.workload != "dgemm":
         {"skinny": bench_skinny, "precision": bench_precision, "serving": bench_serving,
-         "copy": bench_copy, "grouped": bench_grouped, "transpose": bench_transpose,
+         "copy": bench_copy, "mfma": bench_mfma, "grouped": bench_grouped, "transpose": bench_transpose,
          "sgemm512": bench_sgemm512, "sweep": bench_sweep, "stencil": bench_stencil,
          "tune": bench_tune, "batch": bench_batch}[args.workload](args)
         return
@@ -456,6 +456,56 @@ def bench_copy(args):
                   {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                    "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None})
     print(json.dumps(out), flush=True)
+
+
+# Matrix-core forms of the probe: (name, MFMAs per wave per round x flops per MFMA, datasheet peak
+# TFLOP/s, rounds)
+MFMA_FORMS = [("bf16 16x16x32", 32 * 16 * 16 * 32 * 2, 2500.0, 4000),
+              ("bf16 32x32x16", 8 * 32 * 32 * 16 * 2, 2500.0, 4000),
+              ("f16 16x16x32", 32 * 16 * 16 * 32 * 2, 2500.0, 4000),
+              ("f32 16x16x4", 32 * 16 * 16 * 4 * 2, 157.3, 1000),
+              ("f32 32x32x2", 8 * 32 * 32 * 2 * 2, 157.3, 1000),
+              ("f64 16x16x4", 16 * 16 * 16 * 4 * 2, 78.6, 1000),
+              ("fp8 16x16x128 scaled", 16 * 16 * 16 * 128 * 2, 5000.0, 4000)]
+
+
+def bench_mfma(args):
+    """Sustained matrix-core rate per instruction form from registers alone
+    (blasdsp_probe_mfma: one wave per SIMD on every CU, operands FP32 values
+    uniform in [-1, 1) or their 16-bit halves), and global -> LDS load
+    throughput from an L2-resident 2 MiB region per load path
+    (blasdsp_probe_lds).  The ceilings the GEMM kernels are read against."""
+    import torch
+    import __graft_entry__
+
+    blas = __graft_entry__.load_package()
+    stream = torch.cuda.current_stream()
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    src = torch.empty(4096, dtype=torch.float32, device="cuda")
+    blas.fill(src.view(1, -1), 1, 4096, blas.UNIFORM, 5)
+    out = torch.empty(256 * 2 * cus, dtype=torch.float32, device="cuda")
+    forms = {}
+    for f, (name, flops, peak, rounds) in enumerate(MFMA_FORMS):
+        ms = _time_events(lambda: blas.check("blasdsp_probe_mfma", blas.lib().blasdsp_probe_mfma(
+            f, src.data_ptr(), out.data_ptr(), cus, rounds, stream.cuda_stream)), max(3, min(args.steps, 10)), stream)
+        tf = cus * 4 * rounds * flops / ms / 1e9
+        forms[name] = {"tflops": round(tf, 1), "of_datasheet": round(tf / peak, 3)}
+    region = torch.ones(1 << 19, dtype=torch.float32, device="cuda")  # 2 MiB
+    sink = torch.empty(256 * 2 * cus, dtype=torch.int32, device="cuda")
+    loads = {}
+    for mode, name in enumerate(("direct-to-LDS", "registers + ds_write", "registers only")):
+        for per_cu in (1, 2):
+            blocks, iters = per_cu * cus, 2000
+            ms = _time_events(lambda: blas.check("blasdsp_probe_lds", blas.lib().blasdsp_probe_lds(
+                mode, region.data_ptr(), sink.data_ptr(), blocks, iters, stream.cuda_stream)),
+                max(3, min(args.steps, 10)), stream)
+            loads["%s, %d workgroup(s) per CU" % (name, per_cu)] = round(blocks * iters * 32768 / ms / 1e9, 2)
+    out_line = _common(args, forms["f64 16x16x4"]["tflops"], "TFLOP/s", "f64",
+                       {"workload": "register-only MFMA streams, one wave per SIMD on %d CUs" % cus,
+                        "mfma_forms": forms, "lds_load_tbs": loads},
+                       {"bound": "mfma", "achieved": forms["f64 16x16x4"]["tflops"], "peak": 78.6,
+                        "unit": "TFLOP/s", "frac": forms["f64 16x16x4"]["of_datasheet"], "traffic": None})
+    print(json.dumps(out_line), flush=True)
 
 
 def bench_grouped(args):

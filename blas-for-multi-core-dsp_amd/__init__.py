@@ -149,6 +149,8 @@ def _declare(L: ctypes.CDLL) -> None:
     L.blasdsp_tune_load.argtypes = [ctypes.c_char_p]
     L.blasdsp_tune_clear.argtypes = []
     L.blasdsp_probe_read.argtypes = [_vp, _i64, _vp, ctypes.c_int, _vp]
+    L.blasdsp_probe_mfma.argtypes = [ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp]
+    L.blasdsp_probe_lds.argtypes = [ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp]
     L.blasdsp_stream_release.argtypes = [_vp]
     for name in ("transpose_16x16_dbuf", "transpose_float_32x32_dbuf", "transpose_null", "transpose_float_null",
                  "transpose_A2B_float32"):

@@ -489,6 +489,24 @@ BLASDSP_API int blasdsp_stream_release(void *stream);
  * bandwidth-bound skinny kernels are measured against. */
 BLASDSP_API int blasdsp_probe_read(const void *p, int64_t bytes, void *out, int blocks, void *stream);
 
+/* The sustained rate of one matrix-core instruction form from registers
+ * alone: `blocks` workgroups of 4 waves, each wave loading its operands once
+ * from src (device, 16 KiB, 16-byte aligned: 1024 words of 16 bytes) and then
+ * issuing `iters` rounds of independent MFMAs; out (device) receives
+ * 256 * blocks floats.  Per wave and round: form 0 BF16 16x16x32 and 2 FP16
+ * 16x16x32, 32 MFMAs; 1 BF16 32x32x16 and 4 FP32 32x32x2, 8 MFMAs; 3 FP32
+ * 16x16x4, 32 MFMAs; 5 FP64 16x16x4 and 6 scaled FP8 16x16x128 (unit
+ * scales), 16 MFMAs.  The caller times the launch. */
+BLASDSP_API int blasdsp_probe_mfma(int form, const void *src, void *out, int blocks, int iters, void *stream);
+
+/* Global -> LDS operand-load throughput from an L2-resident region: `blocks`
+ * workgroups of 4 waves each load 32 KiB per iteration from src (device, 2 MiB)
+ * into LDS, the previous iteration's loads still in flight; mode 0
+ * direct-to-LDS buffer loads, 1 loads into registers and ds_write, 2 loads
+ * into registers only.  out (device) receives 256 * blocks words.  The
+ * caller times the launch (bytes = blocks * iters * 32 KiB). */
+BLASDSP_API int blasdsp_probe_lds(int mode, const void *src, void *out, int blocks, int iters, void *stream);
+
 /* ---- synthetic inputs ----------------------------------------------------
  * Fill the logical rows x cols matrix stored at X (device memory, leading
  * dimension ld, transposed storage when trans != 0) from the counter-based
