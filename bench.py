@@ -481,15 +481,27 @@ def bench_mfma(args):
     blas = __graft_entry__.load_package()
     stream = torch.cuda.current_stream()
     cus = torch.cuda.get_device_properties(0).multi_processor_count
-    src = torch.empty(4096, dtype=torch.float32, device="cuda")
-    blas.fill(src.view(1, -1), 1, 4096, blas.UNIFORM, 5)
+    import numpy as np
+
+    # operand data: FP32 uniform in [-1, 1) from the library's generator (the headline), constant
+    # 0x3c bytes, and random bits with bit 6 of every byte clear (finite, small exponents)
+    uniform = torch.empty(4096, dtype=torch.float32, device="cuda")
+    blas.fill(uniform.view(1, -1), 1, 4096, blas.UNIFORM, 5)
+    const = torch.full((4096,), 0x3c3c3c3c, dtype=torch.int32, device="cuda")
+    bits = np.random.default_rng(5).integers(0, 2**32, 4096, dtype=np.uint64).astype(np.uint32) & 0xBFBFBFBF
+    rand = torch.from_numpy(bits.view(np.int32)).to("cuda")
     out = torch.empty(256 * 2 * cus, dtype=torch.float32, device="cuda")
-    forms = {}
-    for f, (name, flops, peak, rounds) in enumerate(MFMA_FORMS):
-        ms = _time_events(lambda: blas.check("blasdsp_probe_mfma", blas.lib().blasdsp_probe_mfma(
-            f, src.data_ptr(), out.data_ptr(), cus, rounds, stream.cuda_stream)), max(3, min(args.steps, 10)), stream)
-        tf = cus * 4 * rounds * flops / ms / 1e9
-        forms[name] = {"tflops": round(tf, 1), "of_datasheet": round(tf / peak, 3)}
+    forms, by_data = {}, {}
+    for data, src in (("uniform", uniform), ("constant", const), ("random_bits", rand)):
+        rates = {}
+        for f, (name, flops, peak, rounds) in enumerate(MFMA_FORMS):
+            ms = _time_events(lambda: blas.check("blasdsp_probe_mfma", blas.lib().blasdsp_probe_mfma(
+                f, src.data_ptr(), out.data_ptr(), cus, rounds, stream.cuda_stream)), max(3, min(args.steps, 10)),
+                stream)
+            tf = cus * 4 * rounds * flops / ms / 1e9
+            rates[name] = {"tflops": round(tf, 1), "of_datasheet": round(tf / peak, 3)}
+        by_data[data] = rates
+    forms = by_data["uniform"]
     region = torch.ones(1 << 19, dtype=torch.float32, device="cuda")  # 2 MiB
     sink = torch.empty(256 * 2 * cus, dtype=torch.int32, device="cuda")
     loads = {}
@@ -502,7 +514,8 @@ def bench_mfma(args):
             loads["%s, %d workgroup(s) per CU" % (name, per_cu)] = round(blocks * iters * 32768 / ms / 1e9, 2)
     out_line = _common(args, forms["f64 16x16x4"]["tflops"], "TFLOP/s", "f64",
                        {"workload": "register-only MFMA streams, one wave per SIMD on %d CUs" % cus,
-                        "mfma_forms": forms, "lds_load_tbs": loads},
+                        "mfma_forms": forms, "mfma_forms_constant_data": by_data["constant"],
+                        "mfma_forms_random_bits": by_data["random_bits"], "lds_load_tbs": loads},
                        {"bound": "mfma", "achieved": forms["f64 16x16x4"]["tflops"], "peak": 78.6,
                         "unit": "TFLOP/s", "frac": forms["f64 16x16x4"]["of_datasheet"], "traffic": None})
     print(json.dumps(out_line), flush=True)
