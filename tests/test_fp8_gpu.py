@@ -104,6 +104,31 @@ def test_fp8_quantizer_every_magnitude(orc, blas, dev):
     np.testing.assert_array_equal(C.cpu().numpy()[:, 0], want)
 
 
+def test_fp8_quantizer_rounding_boundaries(orc, blas, dev):
+    """Every tie between adjacent E4M3 codes, and the FP32 values on either
+    side of it, of both signs and through the subnormals: the quantizer rounds
+    each to nearest, ties to even, as the oracle's encoder does."""
+    import torch
+
+    codes = np.arange(0, 0x7F)  # 0 .. 448, positive finite codes
+    vals = np.array([orc.fp8_decode(int(q)) for q in codes], np.float64)
+    mids = ((vals[:-1] + vals[1:]) / 2).astype(np.float32)  # exact in FP32
+    v = np.concatenate([mids, np.nextafter(mids, np.float32(0)), np.nextafter(mids, np.float32(np.inf)),
+                        vals.astype(np.float32)])
+    v = np.concatenate([v, -v]).astype(np.float32)
+    M = v.size
+    A = np.zeros((M, 128), np.float32)
+    A[:, 0] = v
+    A[:, 127] = 448.0
+    B = np.zeros((128, 1), np.float32)
+    B[0, 0] = 1.0
+    C = torch.zeros((M, 1), dtype=torch.float32, device=dev)
+    blas.sgemm_fp8("N", "N", to_dev(A, dev), to_dev(B, dev), C, 1.0, 0.0)
+    torch.cuda.synchronize()
+    want = np.array([orc.fp8_decode(orc.fp8_e4m3(float(x))) for x in v], np.float32)
+    np.testing.assert_array_equal(C.cpu().numpy()[:, 0], want)
+
+
 def test_fp8_group_sum_quantum(orc, blas, dev):
     """The property the general bar rests on: within one group of 32 k the
     sum 448 + s loses at most 2^-5 = 2^-13 * 2^8; across two groups it is
