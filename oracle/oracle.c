@@ -412,6 +412,70 @@ double oracle_normwise_f32(int64_t M, int64_t N, const float *X, int64_t ldx,
     return (double)sqrtl(num / den);
 }
 
+/* ---- random projection -----------------------------------------------------
+ * Y = op(A) X with long-double sums, rounded once: op(A) is M x K (A stored
+ * K x M when trans), X is K x S and Y is M x S, both row-major with leading
+ * dimension S.  The checker for whole GEMM results: a result G of
+ * R = alpha op(A) op(B) + beta C0 is compared through G X against
+ * R X = alpha op(A) (op(B) X) + beta C0 X for random X (Freivalds' method),
+ * O((M + K) N S) work where the full product would cost O(M N K). */
+#define PROJ_MAX_S 16
+#define PROJ_COLS 32
+
+static inline long double proj_elem(const void *A, int is32, int64_t idx)
+{
+    return is32 ? (long double)((const float *)A)[idx] : (long double)((const double *)A)[idx];
+}
+
+static int project(int trans, int64_t M, int64_t K, int64_t S, const void *A, int is32, int64_t lda,
+                   const double *X, double *Y)
+{
+    if (S < 1 || S > PROJ_MAX_S || M < 0 || K < 0) return -1;
+    if (!trans) {
+#pragma omp parallel for num_threads(oracle_get_threads()) schedule(static)
+        for (int64_t i = 0; i < M; i++) {
+            long double acc[PROJ_MAX_S] = {0.0L};
+            for (int64_t k = 0; k < K; k++) {
+                const long double a = proj_elem(A, is32, IDX(i, k, lda));
+                for (int64_t s = 0; s < S; s++) acc[s] += a * (long double)X[IDX(k, s, S)];
+            }
+            for (int64_t s = 0; s < S; s++) Y[IDX(i, s, S)] = (double)acc[s];
+        }
+        return 0;
+    }
+    /* stored K x M: each thread takes PROJ_COLS consecutive columns and walks
+     * down the rows, so every row segment it reads is contiguous */
+    const int64_t nblk = (M + PROJ_COLS - 1) / PROJ_COLS;
+#pragma omp parallel for num_threads(oracle_get_threads()) schedule(dynamic)
+    for (int64_t b = 0; b < nblk; b++) {
+        const int64_t i0 = b * PROJ_COLS;
+        const int64_t nc = M - i0 < PROJ_COLS ? M - i0 : PROJ_COLS;
+        long double acc[PROJ_COLS][PROJ_MAX_S];
+        for (int64_t c = 0; c < nc; c++)
+            for (int64_t s = 0; s < S; s++) acc[c][s] = 0.0L;
+        for (int64_t k = 0; k < K; k++)
+            for (int64_t c = 0; c < nc; c++) {
+                const long double a = proj_elem(A, is32, IDX(k, i0 + c, lda));
+                for (int64_t s = 0; s < S; s++) acc[c][s] += a * (long double)X[IDX(k, s, S)];
+            }
+        for (int64_t c = 0; c < nc; c++)
+            for (int64_t s = 0; s < S; s++) Y[IDX(i0 + c, s, S)] = (double)acc[c][s];
+    }
+    return 0;
+}
+
+int oracle_project_f64(char trans, int64_t M, int64_t K, int64_t S, const double *A, int64_t lda,
+                       const double *X, double *Y)
+{
+    return project(trans == 'T' || trans == 't', M, K, S, A, 0, lda, X, Y);
+}
+
+int oracle_project_f32(char trans, int64_t M, int64_t K, int64_t S, const float *A, int64_t lda,
+                       const double *X, double *Y)
+{
+    return project(trans == 'T' || trans == 't', M, K, S, A, 1, lda, X, Y);
+}
+
 /* ---- scaled FP8 (E4M3) GEMM ------------------------------------------------
  * No reference counterpart: the FP8 form of the FP32 GEMM.  The rule, shared
  * with the library's quantizer:

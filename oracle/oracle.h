@@ -110,6 +110,15 @@ double oracle_normwise_f64(int64_t M, int64_t N, const double *X, int64_t ldx,
 double oracle_normwise_f32(int64_t M, int64_t N, const float *X, int64_t ldx,
                            const double *R, int64_t ldr, double *maxabs);
 
+/* Y = op(A) X in extended precision (op(A) M x K, A stored K x M when trans;
+ * X K x S and Y M x S row-major with leading dimension S, 1 <= S <= 16):
+ * the random projection that checks whole GEMM results.  Returns 0, or -1 for
+ * S out of range.  No reference counterpart. */
+int oracle_project_f64(char trans, int64_t M, int64_t K, int64_t S, const double *A, int64_t lda,
+                       const double *X, double *Y);
+int oracle_project_f32(char trans, int64_t M, int64_t K, int64_t S, const float *A, int64_t lda,
+                       const double *X, double *Y);
+
 /* Scaled FP8 (E4M3, one power-of-two scale per 128-element block of each row
  * of op(A) and column of op(B)); see oracle.c for the rule.  No reference
  * counterpart. */

@@ -65,6 +65,9 @@ def lib() -> ctypes.CDLL:
         L.oracle_normwise_f64.argtypes = [_i64, _i64, _dp, _i64, _dp, _i64, ctypes.POINTER(ctypes.c_double)]
         L.oracle_normwise_f32.restype = ctypes.c_double
         L.oracle_normwise_f32.argtypes = [_i64, _i64, _dp, _i64, _dp, _i64, ctypes.POINTER(ctypes.c_double)]
+        for f in (L.oracle_project_f64, L.oracle_project_f32):
+            f.restype = ctypes.c_int
+            f.argtypes = [ctypes.c_char, _i64, _i64, _i64, _dp, _i64, _dp, _dp]
         L.oracle_fp8_e4m3.restype = ctypes.c_uint
         L.oracle_fp8_e4m3.argtypes = [ctypes.c_float]
         L.oracle_fp8_decode.restype = ctypes.c_float
@@ -167,6 +170,62 @@ def gemm_xp(transA: str, transB: str, A, B, C, alpha: float, beta: float) -> np.
         lib().oracle_gemm_xp_f32(transA.encode(), transB.encode(), M, N, K, alpha, _ptr(A), _ld(A),
                                  _ptr(B), _ld(B), beta, _ptr(C), _ld(C), _ptr(out), N)
     return out
+
+
+def project(trans: str, A, X) -> np.ndarray:
+    """op(A) X with extended-precision sums, rounded once to float64: A float32
+    or float64 with contiguous rows (op(A) = A^T when trans == "T"), X a
+    float64 K x S array with 1 <= S <= 16."""
+    M = A.shape[1] if trans == "T" else A.shape[0]
+    K = A.shape[0] if trans == "T" else A.shape[1]
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    assert X.shape[0] == K and 1 <= X.shape[1] <= 16, (X.shape, K)
+    Y = np.empty((M, X.shape[1]), dtype=np.float64)
+    fn = lib().oracle_project_f64 if A.dtype == np.float64 else lib().oracle_project_f32
+    assert A.dtype in (np.float64, np.float32)
+    rc = fn(trans.encode(), M, K, X.shape[1], _ptr(A), _ld(A), _ptr(X), _ptr(Y))
+    assert rc == 0, rc
+    return Y
+
+
+def gemm_projection(transA: str, A, transB: str, B, C0, alpha: float, beta: float, X,
+                    side: str = "right") -> np.ndarray:
+    """The random projection of R = alpha op(A) op(B) + beta C0, without forming R:
+    side "right": R X = alpha op(A) (op(B) X) + beta C0 X (X is N x S, result M x S);
+    side "left": R^T X = alpha op(B)^T (op(A)^T X) + beta C0^T X (X is M x S, result N x S)."""
+    flip = {"N": "T", "T": "N"}
+    if side == "right":
+        RX = alpha * project(transA, A, project(transB, B, X))
+        c = ("N", C0)
+    else:
+        RX = alpha * project(flip[transB], B, project(flip[transA], A, X))
+        c = ("T", C0)
+    if beta != 0.0:
+        RX += beta * project(c[0], c[1], X)
+    return RX
+
+
+def projected_errors(G, RX, X, side: str = "right") -> tuple[float, float, int]:
+    """Error estimates for a whole GEMM result G from its random projection
+    (Freivalds' method): RX is gemm_projection's for the same X and side, X has
+    independent N(0, 1) entries.  A line is a row of G (side "right") or a
+    column (side "left").  With E = G - R, E[||E X||_F^2] = S ||E||_F^2 and
+    E[||(E X)_i||^2] = S ||E_i||^2 for line i, so returns
+    (||E X||_F / ||R X||_F, the normwise relative error's estimate;
+    max_i ||(E X)_i|| / (sqrt(S) ||G_i||), the worst line's relative error
+    estimate; that line's index).  An error spread over many lines is
+    estimated to a few percent; a rank-one error (one k term lost everywhere)
+    only to the factor chi_S / sqrt(S), within [0.4, 1.8] but for odds below
+    1e-3 at S = 8."""
+    S = X.shape[1]
+    right = side == "right"
+    E = project("N" if right else "T", G, X) - RX
+    den = np.linalg.norm(RX)
+    nw = float(np.linalg.norm(E) / den) if den > 0 else (0.0 if not E.any() else np.inf)
+    g = np.sqrt(np.einsum("ij,ij->i" if right else "ij,ij->j", G, G, dtype=np.float64))
+    line = np.linalg.norm(E, axis=1) / (np.sqrt(S) * np.maximum(g, np.finfo(np.float64).tiny))
+    i = int(np.argmax(line))
+    return nw, float(line[i]), i
 
 
 def gemm_fp8(transA: str, transB: str, A, B, C, alpha: float, beta: float) -> np.ndarray:

@@ -12,10 +12,17 @@ against the extended-precision oracle.
       alpha = beta = 1; N(0,1) (long operand seed 2, small seed 3, C seed 4).
       Operands are generated on the device; a 96-wide band of C (columns for
       shortA, rows for thinB) is checked against the oracle on the band's
-      operands copied back from the device.
+      operands copied back from the device, and the whole C through the
+      oracle's random projection from the long side.
   c4  SGEMM NN 8192^3, U[-1,1) and log-uniform 2^[-8,8] (seeds 3, 4, 5):
       plain FP32, split-BF16 with 6 and 3 terms, split-FP16 and scaled FP8;
-      a 32-row band of C against the oracle, each form with its bar.
+      a 32-row band of C against the oracle and the whole C through the
+      oracle's random projection, each form with its bar.
+
+Random projection (Freivalds' method, S = 8 Gaussian vectors): C X against
+R X = alpha op(A) (op(B) X) + beta C0 X in extended precision, for every
+line of C at once; the normwise estimate takes the form's bar, every line's
+relative error estimate four times it (FP64: 1e-12).
 
 Bars (normwise ||C - C_ref||_F / ||C_ref||_F):
   FP64 1e-13; FP32 (sqrt(K) + 4) 2^-23; split-BF16 6 terms and split-FP16
@@ -31,6 +38,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 TOL64 = 1e-13
+LINE64 = 1e-12
+PROJ_S = 8
 
 
 def tol32(K):
@@ -97,10 +106,12 @@ def c3_cases():
 _long_cache = {}
 
 
-def long_operand(blas, dev, prec, kind, trans):
+def long_operand(orc, blas, dev, prec, kind, trans):
     """The streamed operand (shortA: op(B), K x LONG; thinB: op(A), LONG x K),
     stored transposed when trans; generated once per (precision, kind,
-    orientation) and kept while consecutive cases use it."""
+    orientation) and kept while consecutive cases use it, with its random
+    projection from the long side: op(B) X for shortA, op(A)^T X for thinB
+    (X: LONG x S, N(0,1), seed 43)."""
     import torch
 
     key = (prec, kind, trans)
@@ -111,7 +122,14 @@ def long_operand(blas, dev, prec, kind, trans):
         dt = torch.float64 if prec == "d" else torch.float32
         X = torch.empty((cols, rows) if trans else (rows, cols), dtype=dt, device=dev)
         blas.fill(X, rows, cols, blas.NORMAL, 2, trans=trans)
-        _long_cache[key] = X
+        P = np.random.default_rng(43).standard_normal((LONG, PROJ_S))
+        h = X.cpu().numpy()
+        if kind == "shortA":
+            LP = orc.project("T" if trans else "N", h, P)
+        else:
+            LP = orc.project("N" if trans else "T", h, P)
+        del h
+        _long_cache[key] = (X, P, LP)
     return _long_cache[key]
 
 
@@ -124,12 +142,12 @@ def test_c3_skinny_full_size(orc, blas, dev, prec, kind, mode, small):
     ta, tb = mode[0] == "T", mode[1] == "T"
     if kind == "shortA":
         M, N = small, LONG
-        B = long_operand(blas, dev, prec, kind, tb)
+        B, P, LP = long_operand(orc, blas, dev, prec, kind, tb)
         A = torch.empty((K3, M) if ta else (M, K3), dtype=dt, device=dev)
         blas.fill(A, M, K3, blas.NORMAL, 3, trans=ta)
     else:
         M, N = LONG, small
-        A = long_operand(blas, dev, prec, kind, ta)
+        A, P, LP = long_operand(orc, blas, dev, prec, kind, ta)
         B = torch.empty((N, K3) if tb else (K3, N), dtype=dt, device=dev)
         blas.fill(B, K3, N, blas.NORMAL, 3, trans=tb)
     C = torch.empty((M, N), dtype=dt, device=dev)
@@ -137,6 +155,7 @@ def test_c3_skinny_full_size(orc, blas, dev, prec, kind, mode, small):
     # the band: columns of C for shortA, rows for thinB, placed by case
     idx = c3_cases().index((prec, kind, mode, small))
     start = [0, LONG // 2 - 40, LONG - BAND][idx % 3]
+    C0_full = host(C)
     if kind == "shortA":
         C0 = host(C[:, start:start + BAND])
         opA = host(A.t() if ta else A)                                   # M x K
@@ -150,7 +169,18 @@ def test_c3_skinny_full_size(orc, blas, dev, prec, kind, mode, small):
     got = host(C[:, start:start + BAND] if kind == "shortA" else C[start:start + BAND, :])
     ref = orc.gemm_xp("N", "N", opA, opB, C0, 1.0, 1.0)
     err, _ = orc.normwise(got, ref)
-    assert err <= (TOL64 if prec == "d" else tol32(K3)), err
+    tol = TOL64 if prec == "d" else tol32(K3)
+    assert err <= tol, err
+    # the whole C: R P = op(A) (op(B) P) + C0 P for shortA, R^T P = op(B)^T
+    # (op(A)^T P) + C0^T P for thinB, with the long operand's share cached
+    G = host(C)
+    if kind == "shortA":
+        RP = orc.project("T" if ta else "N", host(A), LP) + orc.project("N", C0_full, P)
+        nw, line, i = orc.projected_errors(G, RP, P)
+    else:
+        RP = orc.project("N" if tb else "T", host(B), LP) + orc.project("T", C0_full, P)
+        nw, line, i = orc.projected_errors(G, RP, P, side="left")
+    assert nw <= tol and line <= (LINE64 if prec == "d" else 4 * tol), (nw, line, i)
 
 
 # ---- c4 --------------------------------------------------------------------
@@ -160,12 +190,14 @@ FORMS = {"fp32": tol32(N4), "bf16x6": 1e-6, "bf16x3": 3e-5, "fp16x3": 1e-6}
 
 
 @pytest.fixture(scope="module")
-def c4_operands(blas, dev):
-    """A, B, C of 8192^2 FP32 on the device for both distributions, and
-    their host copies of a 32-row band of A plus all of B."""
+def c4_operands(orc, blas, dev):
+    """A, B, C of 8192^2 FP32 on the device for both distributions, their
+    host copies, and the projection R X of R = A B + 0.5 C (X: 8192 x S,
+    N(0,1), seed 44)."""
     import torch
 
     out = {}
+    X = np.random.default_rng(44).standard_normal((N4, PROJ_S))
     for name, dist in (("uniform", blas.UNIFORM), ("loguniform", blas.LOGUNIF)):
         A = torch.empty((N4, N4), dtype=torch.float32, device=dev)
         B = torch.empty((N4, N4), dtype=torch.float32, device=dev)
@@ -174,7 +206,11 @@ def c4_operands(blas, dev):
         blas.fill(B, N4, N4, dist, 4)
         blas.fill(C, N4, N4, blas.UNIFORM, 5)
         r0 = N4 // 2 + 8
-        out[name] = (A, B, C, r0, host(A[r0:r0 + ROWS4]), host(B), host(C[r0:r0 + ROWS4]))
+        hA, hB, hC = host(A), host(B), host(C)
+        RX = orc.gemm_projection("N", hA, "N", hB, hC, 1.0, 0.5, X)
+        out[name] = (A, B, C, r0, np.ascontiguousarray(hA[r0:r0 + ROWS4]), hB,
+                     np.ascontiguousarray(hC[r0:r0 + ROWS4]), X, RX)
+        del hA, hC
     yield out
     out.clear()
     torch.cuda.empty_cache()
@@ -185,7 +221,7 @@ def c4_operands(blas, dev):
 def test_c4_precision_study(orc, blas, c4_operands, dist, form):
     import torch
 
-    A, B, C, r0, hA, hB, hC = c4_operands[dist]
+    A, B, C, r0, hA, hB, hC, X, RX = c4_operands[dist]
     D = C.clone()
     alpha, beta = 1.0, 0.5
     if form == "fp32":
@@ -197,11 +233,14 @@ def test_c4_precision_study(orc, blas, c4_operands, dist, form):
     else:
         blas.sgemm_fp8("N", "N", A, B, D, alpha, beta)
     torch.cuda.synchronize()
-    got = host(D[r0:r0 + ROWS4])
+    G = host(D)
     del D
+    got = np.ascontiguousarray(G[r0:r0 + ROWS4])
     exact = orc.gemm_xp("N", "N", hA, hB, hC, alpha, beta)
     err, _ = orc.normwise(got, exact)
-    print(f"c4 {dist} {form}: normwise error vs FP64 {err:.3e}")
+    nw, line, i = orc.projected_errors(G, RX, X)
+    print(f"c4 {dist} {form}: normwise error vs FP64 {err:.3e} (band), {nw:.3e} (whole C, projected); "
+          f"worst row {line:.3e} (row {i})")
     if form == "fp8":
         # the library's quantization of the band's rows and of all of B,
         # accumulated exactly: the band of the full quantized product
@@ -209,5 +248,7 @@ def test_c4_precision_study(orc, blas, c4_operands, dist, form):
         qerr, _ = orc.normwise(got, q)
         assert qerr <= 2.0 ** -11, qerr
         assert 0.002 < err < 0.08, err
+        assert 0.002 < nw < 0.08 and line < 4 * 0.08, (nw, line, i)
     else:
         assert err <= FORMS[form], err
+        assert nw <= FORMS[form] and line <= 4 * FORMS[form], (nw, line, i)

@@ -7,6 +7,12 @@ extended-precision oracle):
         a few ulps for the alpha/beta epilogue.
 Integer-pattern inputs (the reference demos' %11 / %7 matrices) are exact in
 both precisions and are compared bit for bit.
+
+Full-size results are checked whole through the oracle's random projection
+(Freivalds' method, S = 8 Gaussian vectors): the normwise estimate against the
+bars above, and every row's (or column's) relative error estimate against
+1e-12 (FP64) or 4 (sqrt(K) + 4) 2^-23 (FP32) — about 2x for the spread of
+the largest of many chi-square estimates, 2x for rows with cancellation.
 """
 import math
 
@@ -18,10 +24,16 @@ from conftest import to_dev
 pytestmark = pytest.mark.gpu
 
 TOL64 = 1e-13
+LINE64 = 1e-12
+PROJ_S = 8
 
 
 def tol32(K):
     return (math.sqrt(max(K, 1)) + 4) * 2.0 ** -23
+
+
+def line32(K):
+    return 4 * tol32(K)
 
 
 def dtype_np(prec):
@@ -288,8 +300,9 @@ def test_fill_matches_host_generator(orc, blas, dev):
 
 def test_dgemm_4096_all_modes(orc, blas, dev):
     """DGEMM 4096^3 with lda = ldb = ldc = 4104, alpha = 1.5, beta = -0.5,
-    U[-1,1] from seed 1: 32 rows of every mode against the oracle, and the four
-    modes against each other over the whole matrix."""
+    U[-1,1] from seed 1: 32 rows of every mode against the oracle, every row
+    of every mode through the oracle's random projection, and the four modes
+    against each other over the whole matrix."""
     import torch
 
     n, ld = 4096, 4104
@@ -307,13 +320,19 @@ def test_dgemm_4096_all_modes(orc, blas, dev):
         outs[mode] = C[:, :n].cpu().numpy()
         del A, B, C
     r0, nr = 1000, 32
-    A_rows = np.ascontiguousarray(orc.matrix(n, n, 0, 11)[r0:r0 + nr])
+    A_full = orc.matrix(n, n, 0, 11)
     B_full = orc.matrix(n, n, 0, 12)
-    C_rows = np.ascontiguousarray(orc.matrix(n, n, 0, 13)[r0:r0 + nr])
-    ref = orc.gemm_xp("N", "N", A_rows, B_full, C_rows, alpha, beta)
+    C_full = orc.matrix(n, n, 0, 13)
+    ref = orc.gemm_xp("N", "N", np.ascontiguousarray(A_full[r0:r0 + nr]), B_full,
+                      np.ascontiguousarray(C_full[r0:r0 + nr]), alpha, beta)
+    X = np.random.default_rng(41).standard_normal((n, PROJ_S))
+    RX = orc.gemm_projection("N", A_full, "N", B_full, C_full, alpha, beta, X)
     for mode, C in outs.items():
         err, _ = orc.normwise(np.ascontiguousarray(C[r0:r0 + nr]), ref)
         assert err <= TOL64, (mode, err)
+        nw, row, i = orc.projected_errors(C, RX, X)
+        print(f"dgemm 4096 {mode}: projected normwise {nw:.2e}, worst row {row:.2e} (row {i})")
+        assert nw <= TOL64 and row <= LINE64, (mode, nw, row, i)
         merr, _ = orc.normwise(C, outs["NN"])
         assert merr <= 1e-15, (mode, merr)
 
@@ -322,8 +341,9 @@ def test_dgemm_4096_all_modes(orc, blas, dev):
 @pytest.mark.parametrize("kind", ["shortA", "thinB"])
 def test_skinny_full_size(orc, blas, dev, prec, kind):
     """shortA M = 32, K = 8192, N = 65536 (thinB: M and N swapped), N(0,1) from
-    seed 2, alpha = beta = 1: 96 columns (rows) against the oracle; the
-    automatic split-K result against a single-slice run."""
+    seed 2, alpha = beta = 1: 96 columns (rows) against the oracle, the whole
+    result through the oracle's random projection, and the automatic split-K
+    result against a single-slice run."""
     import torch
 
     dt = torch.float64 if prec == "d" else torch.float32
@@ -338,6 +358,7 @@ def test_skinny_full_size(orc, blas, dev, prec, kind):
     j0, nj = 4000, 96
     band = (slice(None), slice(j0, j0 + nj)) if kind == "shortA" else (slice(j0, j0 + nj), slice(None))
     c0 = np.ascontiguousarray(C[band].cpu().numpy())
+    c0_full = C.cpu().numpy()
     C1 = C.clone()
     blas.gemm("N", "N", A, B, C, 1.0, 1.0)
     blas.gemm("N", "N", A, B, C1, 1.0, 1.0, algo=blas.ALGO_SKINNY, splitk=1)
@@ -353,9 +374,17 @@ def test_skinny_full_size(orc, blas, dev, prec, kind):
     ref = orc.gemm_xp("N", "N", a, b, c0, 1.0, 1.0)
     err, _ = orc.normwise(got, ref)
     assert err <= tol, err
-    serr, _ = orc.normwise(np.ascontiguousarray(C.cpu().numpy()),
-                           np.ascontiguousarray(C1.cpu().numpy()).astype(np.float64))
+    got_full = C.cpu().numpy()
+    serr, _ = orc.normwise(got_full, C1.cpu().numpy().astype(np.float64))
     assert serr <= tol, serr
+    # the whole result: every long line (rows of shortA's C, columns of
+    # thinB's) through the projection from the long side
+    side = "right" if kind == "shortA" else "left"
+    X = np.random.default_rng(42).standard_normal((large, PROJ_S))
+    RX = orc.gemm_projection("N", A.cpu().numpy(), "N", B.cpu().numpy(), c0_full, 1.0, 1.0, X, side=side)
+    nw, line, i = orc.projected_errors(got_full, RX, X, side=side)
+    print(f"{prec} {kind}: projected normwise {nw:.2e}, worst line {line:.2e} ({i})")
+    assert nw <= tol and line <= (LINE64 if prec == "d" else line32(K)), (nw, line, i)
 
 
 @pytest.mark.parametrize("ldb", [20_000_000, 23_000_000])

@@ -259,6 +259,126 @@ def test_normwise_matches_numpy(orc):
     assert err32 == pytest.approx(np.linalg.norm(X.astype(np.float32) - R) / np.linalg.norm(R), rel=1e-9)
 
 
+# ---- random projection (whole-result checker) ----------------------------------
+
+@pytest.mark.parametrize("trans", "NT")
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_project_vs_fractions(orc, trans, dt):
+    rng = np.random.default_rng(21)
+    M, K, S = 5, 7, 3
+    Aop = rng.standard_normal((M, K)).astype(dt)
+    X = rng.standard_normal((K, S))
+    A = np.ascontiguousarray(Aop.T) if trans == "T" else Aop
+    Y = orc.project(trans, A, X)
+    for i in range(M):
+        for s in range(S):
+            ex = float(sum(Fraction(float(Aop[i, k])) * Fraction(X[k, s]) for k in range(K)))
+            assert abs(Y[i, s] - ex) <= np.spacing(abs(ex)), (i, s)
+
+
+@pytest.mark.parametrize("trans", "NT")
+def test_project_padded_blocks_and_limits(orc, trans):
+    """Row padding, a column count that leaves a partial block of the
+    transposed walk (70 = 2 x 32 + 6), every S from 1 to 16, and S = 17
+    refused."""
+    rng = np.random.default_rng(22)
+    M, K = 70, 45
+    Aop = rng.standard_normal((M, K))
+    store = np.zeros((K, M + 5) if trans == "T" else (M, K + 3))
+    A = store[:, :M] if trans == "T" else store[:, :K]
+    A[...] = Aop.T if trans == "T" else Aop
+    for S in (1, 7, 16):
+        X = rng.standard_normal((K, S))
+        np.testing.assert_allclose(orc.project(trans, A, X), Aop @ X, rtol=1e-12, atol=1e-13)
+    with pytest.raises(AssertionError):
+        orc.project(trans, A, rng.standard_normal((K, 17)))
+
+
+def _projection_case(orc, M, N, K, seed=23, S=8):
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((K, N))
+    C0 = rng.standard_normal((M, N))
+    R = orc.gemm_xp("N", "N", A, B, C0, 1.5, -0.5)
+    X = rng.standard_normal((N, S))
+    RX = orc.gemm_projection("N", A, "N", B, C0, 1.5, -0.5, X)
+    return A, B, C0, R, X, RX, rng
+
+
+def test_projected_errors_estimate_the_normwise_error(orc):
+    """Planted errors of known size: the estimate matches the directly
+    measured normwise error; an exact result reads at rounding level."""
+    A, B, C0, R, X, RX, rng = _projection_case(orc, 300, 200, 150)
+    nw, row, _ = orc.projected_errors(R, RX, X)
+    assert nw < 1e-15 and row < 1e-14, (nw, row)
+    G = R + 1e-8 * np.abs(R).mean() * rng.standard_normal(R.shape)
+    true, _ = orc.normwise(G, R)
+    nw, _, _ = orc.projected_errors(G, RX, X)
+    assert abs(nw / true - 1) < 0.1, (nw, true)
+    # one k term dropped from every output: a rank-one error, whose projection
+    # has only S degrees of freedom (estimate = true x chi_S / sqrt(S))
+    G = 1.5 * (A[:, 1:] @ B[1:, :]) - 0.5 * C0
+    true, _ = orc.normwise(G, R)
+    nw, _, _ = orc.projected_errors(G, RX, X)
+    assert true > 1e-3 and 0.4 < nw / true < 1.8, (nw, true)
+    # op(A) transposed by mistake (square case)
+    A2, B2, C2, R2, X2, RX2, _ = _projection_case(orc, 120, 90, 120, seed=24)
+    nw, _, _ = orc.projected_errors(1.5 * (A2.T @ B2) - 0.5 * C2, RX2, X2)
+    assert nw > 0.3, nw
+
+
+def test_projected_errors_find_one_wrong_element(orc):
+    """A single output off by 1e-9 of its row's norm is found in its row by the
+    right projection and in its column by the left one, while every other
+    row reads at rounding level."""
+    A, B, C0, R, X, RX, rng = _projection_case(orc, 300, 200, 150)
+    i0, j0 = 217, 41
+    G = R.copy()
+    G[i0, j0] += 1e-9 * np.linalg.norm(R[i0])
+    nw, row, i = orc.projected_errors(G, RX, X)
+    assert i == i0 and row > 2e-10, (i, row)
+    E = orc.project("N", G, X) - RX
+    mask = np.arange(R.shape[0]) != i0
+    others = np.linalg.norm(E[mask], axis=1) / (np.sqrt(X.shape[1]) * np.linalg.norm(G[mask], axis=1))
+    assert others.max() < 1e-14, others.max()
+    Y = rng.standard_normal((R.shape[0], 8))
+    RY = orc.gemm_projection("N", A, "N", B, C0, 1.5, -0.5, Y, side="left")
+    np.testing.assert_allclose(RY, R.T @ Y, rtol=1e-12, atol=1e-10)
+    nw_l, col, j = orc.projected_errors(R, RY, Y, side="left")
+    assert nw_l < 1e-15 and col < 1e-14, (nw_l, col)
+    G[i0, j0] = R[i0, j0] + 1e-9 * np.linalg.norm(R[:, j0])
+    _, col, j = orc.projected_errors(G, RY, Y, side="left")
+    assert j == j0 and col > 2e-10, (j, col)
+
+
+def test_gemm_projection_modes_and_f32(orc):
+    """Every storage mode of A and B gives the same R X; FP32 operands
+    project exactly as their FP64 values do."""
+    rng = np.random.default_rng(25)
+    M, N, K, S = 40, 33, 57, 4
+    Aop = rng.standard_normal((M, K)).astype(np.float32)
+    Bop = rng.standard_normal((K, N)).astype(np.float32)
+    C0 = rng.standard_normal((M, N)).astype(np.float32)
+    X = rng.standard_normal((N, S))
+    R = orc.gemm_xp("N", "N", Aop, Bop, C0, 2.0, 0.25)
+    want = R @ X
+    base = orc.gemm_projection("N", Aop.astype(np.float64), "N", Bop.astype(np.float64), C0.astype(np.float64),
+                               2.0, 0.25, X)
+    Y = rng.standard_normal((M, S))
+    base_l = orc.gemm_projection("N", Aop, "N", Bop, C0, 2.0, 0.25, Y, side="left")
+    want_l = R.T @ Y
+    for ta in "NT":
+        for tb in "NT":
+            A = np.ascontiguousarray(Aop.T) if ta == "T" else Aop
+            B = np.ascontiguousarray(Bop.T) if tb == "T" else Bop
+            RX = orc.gemm_projection(ta, A, tb, B, C0, 2.0, 0.25, X)
+            np.testing.assert_array_equal(RX, base)
+            assert np.linalg.norm(RX - want) / np.linalg.norm(want) < 1e-14
+            RY = orc.gemm_projection(ta, A, tb, B, C0, 2.0, 0.25, Y, side="left")
+            np.testing.assert_array_equal(RY, base_l)
+            assert np.linalg.norm(RY - want_l) / np.linalg.norm(want_l) < 1e-14
+
+
 def test_mode_invariance_with_padding(orc):
     # the reference harness builds explicit transposed copies and checks all
     # four modes against one NN result (dgemm_release_V1.1/host_code/dgemm_demo.c:74,82,101-131)
