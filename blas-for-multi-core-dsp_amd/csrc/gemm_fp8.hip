@@ -10,17 +10,22 @@
 //       contiguous or not, whichever way they are stored), pick each 128-block's
 //       exponent e (the smallest with amax 2^-e <= 448), and write the E4M3
 //       codes of x 2^-e (round to nearest even) K-contiguously, zero-padded to
-//       128-multiples in every dimension, plus the biased exponents 127 + e
+//       whole tiles in every dimension, plus the biased exponents 127 + e
 //       (E8M0 scale bytes);
-//   fp8_big_kernel  128 x 128 output tiles, K steps of 128 = one
-//       v_mfma_scale_f32_16x16x128_f8f6f4 depth, so each step's scales are
-//       one byte per row / column, applied by the matrix core itself; the
-//       quantized operands go global -> LDS by direct-to-LDS loads into an
-//       XOR-swizzled layout, two workgroups per CU.  At 8192^3 on MI355X the
-//       whole form (quantization included) went from 0.88-0.90 ms with
-//       fp8_gemm_kernel (the same tiles staged through registers into
-//       padded LDS rows) to 0.74-0.76 ms, bit-identical; 256 x 128 and
-//       256 x 256 tiles at one workgroup per CU ran slower (0.97-1.0 ms).
+//   fp8_big_kernel  K steps of 128 = one v_mfma_scale_f32_16x16x128_f8f6f4
+//       depth, so each step's scales are one byte per row / column, applied
+//       by the matrix core itself; the quantized operands and each wave's
+//       scale words go global -> LDS by direct-to-LDS loads, the operands into
+//       an XOR-swizzled layout.  128 x 128 tiles of 4 waves, two workgroups
+//       per CU; 256 x 256 tiles of 8 waves, one per CU, when a launch has at
+//       least four of those per CU.  At 8192^3 on MI355X the whole form
+//       (quantization included) went from 0.88-0.90 ms with fp8_gemm_kernel
+//       (the same tiles staged through registers into padded LDS rows) to
+//       0.74-0.76 ms with direct-to-LDS loads; 256 x 128 and 256 x 256 tiles
+//       of 4 waves at one workgroup per CU ran 0.97-1.0 ms.  On one box, all
+//       bit-identical: the scale words through LDS in place of registers
+//       (whose loads the compiler waited on at every loop entry) 0.77-0.79 ->
+//       0.73-0.76 ms, 256 x 256 tiles of 8 waves 0.69-0.74.
 //       fp8_gemm_kernel remains for K past the direct loads' 32-bit lane
 //       offsets (8 Kp >= 2^31).
 //
@@ -31,6 +36,7 @@
 #include <math.h>
 
 #include <type_traits>
+#include <utility>
 
 #include "api_common.h"
 
@@ -289,44 +295,73 @@ fp8_gemm_kernel(const uint8_t *__restrict__ Qa, const uint8_t *__restrict__ Qb, 
             }
 }
 
-// The direct-to-LDS GEMM, TM x TN tiles of 4 waves ((TM / 2) x (TN / 2)
-// each).  Per K step of 128 a workgroup needs TM + TN rows x 128 bytes; they
-// go global -> LDS by direct-to-LDS buffer loads (lane l's 16 bytes land at
-// the block base + 16 l), 1 KiB each, into two LDS stages, one step ahead.  A
-// 1 KiB block holds 8 rows; the 16-byte chunk c of row r sits in slot 8 r +
-// (c ^ (r & 7)), so lane l of a load reads row l / 8, chunk (l % 8) ^ (l / 8),
-// and a fragment read (16 rows, the same chunk) touches 8 distinct bank groups
-// per 8 lanes: no bank conflicts.
-template <int TM, int TN>
+// The direct-to-LDS GEMM, TM x TN tiles of NW waves in a 2 x (NW / 2) grid
+// ((TM / 2) x (2 TN / NW) each).  Per K step of 128 a workgroup needs TM + TN
+// rows x 128 bytes; they go global -> LDS by direct-to-LDS buffer loads (lane
+// l's 16 bytes land at the block base + 16 l), 1 KiB each, into a ring of NS
+// LDS stages, NS - 1 steps ahead.  A 1 KiB block holds 8 rows; the 16-byte
+// chunk c of row r sits in slot 8 r + (c ^ (r & 7)), so lane l of a load
+// reads row l / 8, chunk (l % 8) ^ (l / 8), and a fragment read (16 rows, the
+// same chunk) touches 8 distinct bank groups per 8 lanes: no bank conflicts.
+// Each wave's scale words of a step (its rows' and its columns' 64-row
+// groups; one word holds the scales of four 16-row fragments) go to the
+// wave's own 512 bytes of the stage by two more direct-to-LDS loads, so no
+// register waits on a load across steps.
+template <int TM, int TN, int NW, int NS>
 struct BigTile {
     static constexpr int BLOCKS_A = TM / 8, BLOCKS = (TM + TN) / 8;  // 1 KiB blocks per stage
-    static constexpr int LOADS = BLOCKS / 4;                         // per wave
-    static constexpr int MI = TM / 32, NJ = TN / 32;                 // MFMA tiles per wave
-    static constexpr int WPE = BLOCKS * 2 <= 80 ? 2 : 1;             // workgroups per CU the LDS allows
+    static constexpr int LOADS = BLOCKS / NW;                        // per wave
+    static constexpr int WC = NW / 2;                                // wave columns
+    static constexpr int MI = TM / 32, NJ = TN / (16 * WC);          // MFMA tiles per wave
+    static constexpr int GA = (MI + 3) / 4, GB = (NJ + 3) / 4;       // scale words per lane and operand
+    static constexpr int VM = LOADS + 2;                             // vector-memory ops per stage
+    static constexpr int STAGE = BLOCKS * 1024 + NW * 512;           // bytes per stage
+    static constexpr int WPE = 2 * NS * STAGE <= 160 * 1024 ? 2 : 1;  // workgroups per CU the LDS allows
+    static_assert(BLOCKS % NW == 0 && NJ >= 1 && (TM / 2) % 64 == 0 && (TN / WC) % 64 == 0, "tile shape");
+    static_assert(NS >= 2 && NS * STAGE <= 160 * 1024 && (NS - 2) * VM <= 63, "stages");
 };
 
-// s_waitcnt vmcnt(0) through the builtin, so the compiler's own wait
-// insertion sees it (gfx9 encoding; expcnt and lgkmcnt left at their maxima).
-__device__ __forceinline__ void wait_vmcnt0() { __builtin_amdgcn_s_waitcnt((7 << 4) | (15 << 8)); }
+// s_waitcnt vmcnt(N) through the builtin, so the compiler's own wait
+// insertion sees it (gfx9 encoding: vmcnt bits 3:0 and 15:14; expcnt and
+// lgkmcnt left at their maxima).
+template <int N>
+__device__ __forceinline__ void wait_vmcnt()
+{
+    static_assert(N >= 0 && N <= 63, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+// f(integral_constant<int, 0>()), ..., f(integral_constant<int, N - 1>()).
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_seq(F &&f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>()), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    static_for_seq(f, std::make_integer_sequence<int, N>());
+}
 
 // This file is compiled with the MFMAs in their VGPR form (see the
 // Makefile): otherwise, with the register budget of one wave per SIMD, the
 // compiler parks the accumulators in AGPRs and copies them around every MFMA.
-template <int TM, int TN>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BigTile<TM, TN>::WPE, BigTile<TM, TN>::WPE)))
+template <int TM, int TN, int NW, int NS>
+__global__ void __launch_bounds__(64 * NW)
+__attribute__((amdgpu_waves_per_eu(BigTile<TM, TN, NW, NS>::WPE * NW / 4, BigTile<TM, TN, NW, NS>::WPE * NW / 4)))
 fp8_big_kernel(const uint8_t *__restrict__ Qa, const uint8_t *__restrict__ Qb, const uint32_t *__restrict__ Sa,
                const uint32_t *__restrict__ Sb, int64_t Kp, int64_t Mp, int64_t Np, int tiles_m, int tiles_n,
                float *__restrict__ C, int64_t ldc, int64_t M, int64_t N, float alpha, float beta)
 {
-    using Bt = BigTile<TM, TN>;
-    constexpr int MI = Bt::MI, NJ = Bt::NJ;
-    __shared__ __attribute__((aligned(1024))) uint8_t lds0[Bt::BLOCKS * 1024];
-    __shared__ __attribute__((aligned(1024))) uint8_t lds1[Bt::BLOCKS * 1024];
+    using Bt = BigTile<TM, TN, NW, NS>;
+    constexpr int MI = Bt::MI, NJ = Bt::NJ, GA = Bt::GA, GB = Bt::GB;
+    constexpr int kStage = Bt::STAGE;
+    __shared__ __attribute__((aligned(1024))) uint8_t lds[NS * kStage];
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lr = lane & 15, lg = lane >> 4;
-    const int wm = (wave >> 1) * (TM / 2), wn = (wave & 1) * (TN / 2);
+    const int wm = (wave / Bt::WC) * (TM / 2), wn = (wave % Bt::WC) * (TN / Bt::WC);
 
     // XCD-aware grouped tile order, as in the other GEMM kernels.
     int pid = blockIdx.x;
@@ -341,15 +376,18 @@ fp8_big_kernel(const uint8_t *__restrict__ Qa, const uint8_t *__restrict__ Qb, c
     const int nk = (int)(Kp / kQB);
     // lane l of a block load: row l / 8 of the block, chunk (l % 8) ^ (l / 8)
     const uint32_t voff = (uint32_t)((lane >> 3) * Kp + (((lane & 7) ^ (lane >> 3)) * 16));
-    auto stage = [&](auto s_c) -> uint8_t * { return decltype(s_c)::value == 0 ? lds0 : lds1; };
     const int64_t a_groups = Mp / kRowBlock, b_groups = Np / kRowBlock;
     const int64_t a_grp = (m0 + wm) / kRowBlock, b_grp = (n0 + wn) / kRowBlock;
-    constexpr int GA = (MI + 3) / 4, GB = (NJ + 3) / 4;  // scale words (64-row groups) per operand
-    uint32_t sc[2][GA + GB];
+    // lane l of a scale load: word l of the wave's GA (GB) consecutive
+    // 16-word groups; the lanes past them repeat the last word
+    const uint32_t soff_a = (uint32_t)(min(lane, 16 * GA - 1) * 4), soff_b = (uint32_t)(min(lane, 16 * GB - 1) * 4);
+    const int scale_at = Bt::BLOCKS * 1024 + wave * 512;  // this wave's scale words in a stage
+    // Loads of K step kb into stage S: this wave's share of the blocks, then
+    // its scale words (Bt::VM vector-memory operations).
     auto issue = [&](int kb, auto s_c) {
-        constexpr int BB = decltype(s_c)::value;
+        constexpr int S = decltype(s_c)::value;
         const int64_t k = min(kb, nk - 1);
-        uint8_t *dst = stage(s_c);
+        uint8_t *dst = lds + S * kStage;
 #pragma unroll
         for (int q = 0; q < Bt::LOADS; ++q) {
             const int b = wave * Bt::LOADS + q;  // wave-uniform
@@ -361,10 +399,14 @@ fp8_big_kernel(const uint8_t *__restrict__ Qa, const uint8_t *__restrict__ Qb, c
             __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void *)(dst + b * 1024),
                                                      16, voff, 0, 0, 0);
         }
-#pragma unroll
-        for (int g = 0; g < GA; ++g) sc[BB][g] = Sa[(k * a_groups + a_grp + g) * 16 + lr];
-#pragma unroll
-        for (int g = 0; g < GB; ++g) sc[BB][GA + g] = Sb[(k * b_groups + b_grp + g) * 16 + lr];
+        const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<uint32_t *>(Sa + (k * a_groups + a_grp) * 16), (short)0, 0x7fffffff, 0x00020000);
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(ra, (__attribute__((address_space(3))) void *)(dst + scale_at), 4,
+                                                 soff_a, 0, 0, 0);
+        const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<uint32_t *>(Sb + (k * b_groups + b_grp) * 16), (short)0, 0x7fffffff, 0x00020000);
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rb, (__attribute__((address_space(3))) void *)(dst + scale_at + 256),
+                                                 4, soff_b, 0, 0, 0);
     };
 
     f4 acc[MI][NJ];
@@ -373,55 +415,61 @@ fp8_big_kernel(const uint8_t *__restrict__ Qa, const uint8_t *__restrict__ Qb, c
 #pragma unroll
         for (int j = 0; j < NJ; ++j) acc[i][j] = f4{0, 0, 0, 0};
 
-    // Step kt on buffer kt % 2: wait for this wave's loads of stage kt,
-    // barrier (every wave's have landed; every wave is done with the other
-    // buffer), load stage kt + 1 into the other buffer, read this wave's
-    // fragments, issue its MFMAs.  Past the last stage a step reloads the
-    // last one (the same wait every step); that buffer is never read.
-    auto step = [&](int kt, auto b_c) {
-        constexpr int BB = decltype(b_c)::value;
+    // Step kt on stage S = kt % NS: wait for this wave's loads of stage kt
+    // (the NS - 2 later stages' may stay in flight), barrier (every wave's
+    // have landed; every wave is done with the stage step kt - 1 read), load
+    // step kt + NS - 1 into that stage, read this wave's fragments, issue its
+    // MFMAs.  Past the last step a step reloads the last one (the same count
+    // of loads every step); those stages are never read.
+    auto step = [&](int kt, auto s_c) {
+        constexpr int S = decltype(s_c)::value;
         __builtin_amdgcn_sched_barrier(0);
-        wait_vmcnt0();
+        wait_vmcnt<(NS - 2) * Bt::VM>();
         asm volatile("" ::: "memory");
         __builtin_amdgcn_s_barrier();
         asm volatile("" ::: "memory");
         __builtin_amdgcn_sched_barrier(0);
-        issue(kt + 1, std::integral_constant<int, 1 - BB>());
-        const uint8_t *base = stage(b_c);
+        issue(kt + NS - 1, std::integral_constant<int, (S + NS - 1) % NS>());
+        const uint8_t *base = lds + S * kStage;
         auto frag = [&](const uint8_t *op, int row) -> i8v {
             const int c0 = 2 * lg, sw = row & 7;
             const u32x4 x0 = *reinterpret_cast<const u32x4 *>(op + row * 128 + ((c0 ^ sw) * 16));
             const u32x4 x1 = *reinterpret_cast<const u32x4 *>(op + row * 128 + (((c0 + 1) ^ sw) * 16));
             return i8v{(int)x0[0], (int)x0[1], (int)x0[2], (int)x0[3], (int)x1[0], (int)x1[1], (int)x1[2], (int)x1[3]};
         };
-        // op(B)'s fragments stay in registers for the step; op(A)'s are read
-        // one row tile at a time, each used by NJ MFMAs.
+        // op(B)'s fragments and the scale words stay in registers for the
+        // step; op(A)'s fragments are read one row tile at a time, each used
+        // by NJ MFMAs.
+        const uint32_t *sw = reinterpret_cast<const uint32_t *>(base + scale_at);
+        uint32_t sc[GA + GB];
+#pragma unroll
+        for (int g = 0; g < GA; ++g) sc[g] = sw[16 * g + lr];
+#pragma unroll
+        for (int g = 0; g < GB; ++g) sc[GA + g] = sw[64 + 16 * g + lr];
         i8v fb[NJ];
 #pragma unroll
         for (int j = 0; j < NJ; ++j) fb[j] = frag(base + TM * 128, wn + 16 * j + lr);
 #pragma unroll
         for (int i = 0; i < MI; ++i) {
             const i8v fa = frag(base, wm + 16 * i + lr);
-            const int sai = (int)((sc[BB][i >> 2] >> (8 * (i & 3))) & 0xFFu);
+            const int sai = (int)((sc[i >> 2] >> (8 * (i & 3))) & 0xFFu);
 #pragma unroll
             for (int j = 0; j < NJ; ++j) {
-                const int sbj = (int)((sc[BB][GA + (j >> 2)] >> (8 * (j & 3))) & 0xFFu);
+                const int sbj = (int)((sc[GA + (j >> 2)] >> (8 * (j & 3))) & 0xFFu);
                 acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(fa, fb[j], acc[i][j], 0, 0, 0, sai, 0,
                                                                              sbj);
             }
         }
         __builtin_amdgcn_sched_barrier(0);
     };
-    using B0 = std::integral_constant<int, 0>;
-    using B1 = std::integral_constant<int, 1>;
-    issue(0, B0());
+    static_for<NS - 1>([&](auto s_c) { issue(decltype(s_c)::value, s_c); });
+    // NS steps per iteration, so every stage index is a constant
     int kt = 0;
-    for (; kt + 2 <= nk; kt += 2) {
-        step(kt, B0());
-        step(kt + 1, B1());
-    }
-    if (kt < nk) step(kt, B0());
-    wait_vmcnt0();  // no load into this workgroup's LDS may be in flight when it exits
+    for (; kt + NS <= nk; kt += NS) static_for<NS>([&](auto s_c) { step(kt + decltype(s_c)::value, s_c); });
+    static_for<NS>([&](auto s_c) {
+        if (kt + decltype(s_c)::value < nk) step(kt + decltype(s_c)::value, s_c);
+    });
+    wait_vmcnt<0>();  // no load into this workgroup's LDS may be in flight when it exits
 
 #pragma unroll
     for (int i = 0; i < MI; ++i)
@@ -438,8 +486,22 @@ fp8_big_kernel(const uint8_t *__restrict__ Qa, const uint8_t *__restrict__ Qb, c
             }
 }
 
-// Tile of the direct-to-LDS FP8 GEMM.
-constexpr int kBigM = 128, kBigN = 128;
+// Tiles of the direct-to-LDS FP8 GEMM: 128 x 128 with 4 waves (two
+// workgroups per CU) in general, 256 x 256 with 8 waves (one per CU) when a
+// launch has at least four of those per CU.  Measured on MI355X, all
+// bit-identical: at 8192^3 the whole form took 0.69-0.71 ms with 256 tiles
+// against 0.72-0.76 with 128 tiles; at 4096^3 the two were equal, at
+// 2048 x 2048 x 8192 and 3000 x 8192 x 2048 the 256 tiles 10-50 % slower.
+// A ring of 3 or 4 stages at one workgroup per CU ran 0.92-0.93 ms at 8192^3:
+// the second workgroup keeps the matrix cores busy across each barrier.
+constexpr int kBigS = 2;
+constexpr int kWideTile = 256;
+
+int fp8_tile(int64_t M, int64_t N)
+{
+    const int64_t wide = ((M + kWideTile - 1) / kWideTile) * ((N + kWideTile - 1) / kWideTile);
+    return wide >= 4 * (int64_t)num_cus() ? kWideTile : 128;
+}
 
 struct Fp8Layout {
     int64_t Mp, Np, Kp, KB;
@@ -449,11 +511,13 @@ struct Fp8Layout {
 
 size_t up256(size_t b) { return (b + 255) / 256 * 256; }
 
-Fp8Layout fp8_layout(int64_t M, int64_t N, int64_t K, void *base)
+// Quantized operands padded to whole tiles of edge `tile` (a multiple of
+// the 128 of the register-staged kernel).
+Fp8Layout fp8_layout(int64_t M, int64_t N, int64_t K, int tile, void *base)
 {
     Fp8Layout L;
-    L.Mp = (M + kBigM - 1) / kBigM * kBigM;
-    L.Np = (N + kBigN - 1) / kBigN * kBigN;
+    L.Mp = (M + tile - 1) / tile * tile;
+    L.Np = (N + tile - 1) / tile * tile;
     L.KB = (K + kQB - 1) / kQB;
     L.Kp = L.KB * kQB;
     uint8_t *p = (uint8_t *)base;
@@ -490,11 +554,15 @@ hipError_t quantize(const float *X, int64_t ld, bool k_contig, int64_t R, int64_
 
 }  // namespace
 
-size_t fp8_workspace_bytes(int64_t M, int64_t N, int64_t K) { return fp8_layout(M, N, K, nullptr).bytes; }
+size_t fp8_workspace_bytes(int64_t M, int64_t N, int64_t K)
+{
+    return fp8_layout(M, N, K, fp8_tile(M, N), nullptr).bytes;
+}
 
 hipError_t launch_fp8_gemm(const Gemm<float> &g, void *work, hipStream_t s)
 {
-    const Fp8Layout L = fp8_layout(g.M, g.N, g.K, work);
+    const int tile = fp8_tile(g.M, g.N);
+    const Fp8Layout L = fp8_layout(g.M, g.N, g.K, tile, work);
     // op(A) rows are K-contiguous unless A is transposed; op(B) columns are
     // K-contiguous exactly when B is transposed (stored N x K).
     hipError_t e = quantize(g.A, g.lda, !g.ta, g.M, g.K, L.Mp, L.KB, L.Qa, L.Sa, s);
@@ -502,11 +570,17 @@ hipError_t launch_fp8_gemm(const Gemm<float> &g, void *work, hipStream_t s)
     if (e != hipSuccess) return e;
     // the big kernel's lane offsets span 8 rows of the quantized operands
     if (8.0 * (double)L.Kp + 128.0 < 2147483647.0) {
-        if ((L.Mp / kBigM) * (L.Np / kBigN) > 0x7fffffffLL) return hipErrorInvalidValue;
-        const int tiles_m = (int)(L.Mp / kBigM), tiles_n = (int)(L.Np / kBigN);
-        hipLaunchKernelGGL((fp8_big_kernel<kBigM, kBigN>), dim3((unsigned)(tiles_m * tiles_n)), dim3(256), 0, s, L.Qa, L.Qb,
-                           (const uint32_t *)L.Sa, (const uint32_t *)L.Sb, L.Kp, L.Mp, L.Np, tiles_m, tiles_n, g.C,
-                           g.ldc, g.M, g.N, g.alpha, g.beta);
+        if ((L.Mp / tile) * (L.Np / tile) > 0x7fffffffLL) return hipErrorInvalidValue;
+        const int tiles_m = (int)(L.Mp / tile), tiles_n = (int)(L.Np / tile);
+        const dim3 grid((unsigned)(tiles_m * tiles_n));
+        if (tile == kWideTile)
+            hipLaunchKernelGGL((fp8_big_kernel<kWideTile, kWideTile, 8, kBigS>), grid, dim3(512), 0, s, L.Qa, L.Qb,
+                               (const uint32_t *)L.Sa, (const uint32_t *)L.Sb, L.Kp, L.Mp, L.Np, tiles_m, tiles_n, g.C,
+                               g.ldc, g.M, g.N, g.alpha, g.beta);
+        else
+            hipLaunchKernelGGL((fp8_big_kernel<128, 128, 4, kBigS>), grid, dim3(256), 0, s, L.Qa, L.Qb,
+                               (const uint32_t *)L.Sa, (const uint32_t *)L.Sb, L.Kp, L.Mp, L.Np, tiles_m, tiles_n, g.C,
+                               g.ldc, g.M, g.N, g.alpha, g.beta);
         return hipGetLastError();
     }
     if ((L.Mp / kTile) * (L.Np / kTile) > 0x7fffffffLL) return hipErrorInvalidValue;

@@ -204,6 +204,38 @@ def test_fp8_per_block_scales_wide_range(orc, blas, dev):
     assert rel.max() <= TOL_GROUP, rel.max()
 
 
+@pytest.mark.parametrize("mode", ["NN", "NT", "TN", "TT"])
+def test_fp8_wide_tiles_match_small_launches(orc, blas, dev, mode):
+    """M = 8292, N = 8000, K = 300: at least four 256 x 256 tiles per CU, so
+    the 256-tile kernel runs; each 100-row band (top, middle, and the ragged
+    last tile row) equals, bit for bit, the same rows run as a 100-row problem
+    (the 128-tile kernel), and matches the quantized oracle."""
+    import torch
+
+    M, N, K, nb = 8292, 8000, 300, 100
+    ta, tb = mode[0] == "T", mode[1] == "T"
+    A = torch.empty((K, M) if ta else (M, K), dtype=torch.float32, device=dev)
+    B = torch.empty((N, K) if tb else (K, N), dtype=torch.float32, device=dev)
+    C = torch.empty((M, N), dtype=torch.float32, device=dev)
+    blas.fill(A, M, K, blas.LOGUNIF, 81, trans=ta)
+    blas.fill(B, K, N, blas.LOGUNIF, 82, trans=tb)
+    blas.fill(C, M, N, blas.UNIFORM, 83)
+    C0 = C.clone()
+    blas.sgemm_fp8(mode[0], mode[1], A, B, C, 1.5, -0.5)
+    hB = np.ascontiguousarray(B.cpu().numpy())
+    for r0 in (0, 4100, M - nb):
+        Ab = A[:, r0:r0 + nb] if ta else A[r0:r0 + nb]
+        Cb = C0[r0:r0 + nb].clone()
+        blas.sgemm_fp8(mode[0], mode[1], Ab, B, Cb, 1.5, -0.5)
+        torch.cuda.synchronize()
+        got = np.ascontiguousarray(C[r0:r0 + nb].cpu().numpy())
+        np.testing.assert_array_equal(got, Cb.cpu().numpy(), err_msg=f"{mode} rows {r0}")
+        ref = orc.gemm_fp8(mode[0], mode[1], np.ascontiguousarray(Ab.cpu().numpy()), hB,
+                           np.ascontiguousarray(C0[r0:r0 + nb].cpu().numpy()), 1.5, -0.5)
+        err, _ = orc.normwise(got, ref)
+        assert err <= TOL_GROUP, (mode, r0, err)
+
+
 def test_fp8_alpha_zero_and_empty(orc, blas, dev):
     import torch
 
