@@ -300,9 +300,14 @@ fp8_gemm_kernel(const uint8_t *__restrict__ Qa, const uint8_t *__restrict__ Qb, 
 // rows x 128 bytes; they go global -> LDS by direct-to-LDS buffer loads (lane
 // l's 16 bytes land at the block base + 16 l), 1 KiB each, into a ring of NS
 // LDS stages, NS - 1 steps ahead.  A 1 KiB block holds 8 rows; the 16-byte
-// chunk c of row r sits in slot 8 r + (c ^ (r & 7)), so lane l of a load
-// reads row l / 8, chunk (l % 8) ^ (l / 8), and a fragment read (16 rows, the
-// same chunk) touches 8 distinct bank groups per 8 lanes: no bank conflicts.
+// chunk c of row r sits in slot 8 r + (c ^ x(r)), x(r) = 4 r_3 + 2 r_1 + r_2
+// (r_i: bit i of r), so lane l of a load of block b reads row l / 8, chunk
+// (l % 8) ^ (4 (b % 2) + 2 ((l / 16) % 2) + l / 32).  A fragment read (lane
+// l: row l % 16, chunk 2 (l / 16) or 2 (l / 16) + 1) then puts every group
+// of 8, 16 or 32 consecutive lanes on distinct four-byte banks.  Measured
+// on MI355X with a probe of these reads alone: 125 TB/s against 73 TB/s for
+// the swizzles c ^ (r & 7) and c ^ ((r >> 1) & 7) (each of which left
+// SQ_LDS_BANK_CONFLICT at 4 cycles per read) and 39 TB/s with none.
 // Each wave's scale words of a step (its rows' and its columns' 64-row
 // groups; one word holds the scales of four 16-row fragments) go to the
 // wave's own 512 bytes of the stage by two more direct-to-LDS loads, so no
@@ -374,8 +379,12 @@ fp8_big_kernel(const uint8_t *__restrict__ Qa, const uint8_t *__restrict__ Qb, c
     const int64_t m0 = (int64_t)tm * TM, n0 = (int64_t)tn * TN;
 
     const int nk = (int)(Kp / kQB);
-    // lane l of a block load: row l / 8 of the block, chunk (l % 8) ^ (l / 8)
-    const uint32_t voff = (uint32_t)((lane >> 3) * Kp + (((lane & 7) ^ (lane >> 3)) * 16));
+    // lane l of a load of an even (odd) block: row l / 8 of the block, chunk
+    // (l % 8) ^ x with x = 2 ((l / 16) % 2) + l / 32 (plus 4)
+    static_assert(Bt::LOADS % 2 == 0 && Bt::BLOCKS_A % 2 == 0, "block parity is the load index's");
+    const int xl = (((lane >> 4) & 1) << 1) | (lane >> 5);
+    const uint32_t voff[2] = {(uint32_t)((lane >> 3) * Kp + (((lane & 7) ^ xl) * 16)),
+                              (uint32_t)((lane >> 3) * Kp + (((lane & 7) ^ (4 | xl)) * 16))};
     const int64_t a_groups = Mp / kRowBlock, b_groups = Np / kRowBlock;
     const int64_t a_grp = (m0 + wm) / kRowBlock, b_grp = (n0 + wn) / kRowBlock;
     // lane l of a scale load: word l of the wave's GA (GB) consecutive
@@ -397,7 +406,7 @@ fp8_big_kernel(const uint8_t *__restrict__ Qa, const uint8_t *__restrict__ Qb, c
             const __amdgpu_buffer_rsrc_t rs =
                 __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t *>(base), (short)0, 0x7fffffff, 0x00020000);
             __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void *)(dst + b * 1024),
-                                                     16, voff, 0, 0, 0);
+                                                     16, voff[q & 1], 0, 0, 0);
         }
         const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<uint32_t *>(Sa + (k * a_groups + a_grp) * 16), (short)0, 0x7fffffff, 0x00020000);
@@ -432,7 +441,7 @@ fp8_big_kernel(const uint8_t *__restrict__ Qa, const uint8_t *__restrict__ Qb, c
         issue(kt + NS - 1, std::integral_constant<int, (S + NS - 1) % NS>());
         const uint8_t *base = lds + S * kStage;
         auto frag = [&](const uint8_t *op, int row) -> i8v {
-            const int c0 = 2 * lg, sw = row & 7;
+            const int c0 = 2 * lg, sw = (((row >> 3) & 1) << 2) | (((row >> 1) & 1) << 1) | ((row >> 2) & 1);
             const u32x4 x0 = *reinterpret_cast<const u32x4 *>(op + row * 128 + ((c0 ^ sw) * 16));
             const u32x4 x1 = *reinterpret_cast<const u32x4 *>(op + row * 128 + (((c0 + 1) ^ sw) * 16));
             return i8v{(int)x0[0], (int)x0[1], (int)x0[2], (int)x0[3], (int)x1[0], (int)x1[1], (int)x1[2], (int)x1[3]};
