@@ -447,27 +447,41 @@ fp8_big_kernel(const uint8_t *__restrict__ Qa, const uint8_t *__restrict__ Qb, c
             return i8v{(int)x0[0], (int)x0[1], (int)x0[2], (int)x0[3], (int)x1[0], (int)x1[1], (int)x1[2], (int)x1[3]};
         };
         // op(B)'s fragments and the scale words stay in registers for the
-        // step; op(A)'s fragments are read one row tile at a time, each used
-        // by NJ MFMAs.
+        // step; op(A)'s fragments are read kAhead row tiles ahead of the NJ
+        // MFMAs that use them, in the order the group barriers pin: left to
+        // itself the compiler kept two of them and waited on each read right
+        // after issuing it.
         const uint32_t *sw = reinterpret_cast<const uint32_t *>(base + scale_at);
         uint32_t sc[GA + GB];
 #pragma unroll
         for (int g = 0; g < GA; ++g) sc[g] = sw[16 * g + lr];
 #pragma unroll
         for (int g = 0; g < GB; ++g) sc[GA + g] = sw[64 + 16 * g + lr];
-        i8v fb[NJ];
+        i8v fb[NJ], fa[MI];
 #pragma unroll
         for (int j = 0; j < NJ; ++j) fb[j] = frag(base + TM * 128, wn + 16 * j + lr);
+        constexpr int kAhead = MI < 2 ? MI : 2;
+#pragma unroll
+        for (int i = 0; i < kAhead; ++i) fa[i] = frag(base, wm + 16 * i + lr);
 #pragma unroll
         for (int i = 0; i < MI; ++i) {
-            const i8v fa = frag(base, wm + 16 * i + lr);
+            if (i + kAhead < MI) fa[i + kAhead] = frag(base, wm + 16 * (i + kAhead) + lr);
             const int sai = (int)((sc[i >> 2] >> (8 * (i & 3))) & 0xFFu);
 #pragma unroll
             for (int j = 0; j < NJ; ++j) {
                 const int sbj = (int)((sc[GA + (j >> 2)] >> (8 * (j & 3))) & 0xFFu);
-                acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(fa, fb[j], acc[i][j], 0, 0, 0, sai, 0,
+                acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(fa[i], fb[j], acc[i][j], 0, 0, 0, sai, 0,
                                                                              sbj);
             }
+        }
+        // DS reads: the scale words, op(B)'s fragments and the first op(A)
+        // fragments (two 16-byte reads each); then per row tile the reads of
+        // the fragment kAhead ahead and the row tile's MFMAs.
+        __builtin_amdgcn_sched_group_barrier(0x100, 2 + 2 * NJ + 2 * kAhead, 0);
+#pragma unroll
+        for (int i = 0; i < MI; ++i) {
+            if (i + kAhead < MI) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, NJ, 0);
         }
         __builtin_amdgcn_sched_barrier(0);
     };
