@@ -136,7 +136,9 @@ __global__ void __launch_bounds__(256) quant_kc_kernel(const float *__restrict__
 // workgroup takes 64 rows x one k block through LDS.  Loads: thread
 // (r4, kq) = (t % 16, t / 16) reads rows 4 r4 .. + 3 of k rows kq + 16 i as
 // one 16-byte vector (VEC: X 16-byte aligned, ld a multiple of 4); then each
-// thread quantizes 32 consecutive k of one row.
+// thread quantizes 32 consecutive k of one row, four lanes to a row, so a
+// wave's stores are 16 runs of 128 bytes (with a lane to a row, 64 runs of
+// 32 bytes: 86 against 78 us for an 8192 x 8192 operand on MI355X).
 template <bool VEC>
 __global__ void __launch_bounds__(256) quant_mc_kernel(const float *__restrict__ X, int64_t ld, int64_t R,
                                                        int64_t K, int64_t Rp, int64_t KB,
@@ -164,26 +166,32 @@ __global__ void __launch_bounds__(256) quant_mc_kernel(const float *__restrict__
         for (int e = 0; e < 4; ++e) T[4 * r4 + e][k] = v[e];
     }
     __syncthreads();
-    const int rl = tid & 63, w = tid >> 6;
-    const int64_t r = r0 + rl;
-    float amax = 0.0f;
+    // block maxima: thread (rl, w) takes row rl, k w * 32 .. + 31
+    {
+        const int rl = tid & 63, w = tid >> 6;
+        float amax = 0.0f;
 #pragma unroll
-    for (int j = 0; j < 32; ++j) amax = fmaxf(amax, fabsf(T[rl][w * 32 + j]));
-    wmax[w][rl] = amax;
+        for (int j = 0; j < 32; ++j) amax = fmaxf(amax, fabsf(T[rl][w * 32 + j]));
+        wmax[w][rl] = amax;
+    }
     __syncthreads();
-    amax = fmaxf(fmaxf(wmax[0][rl], wmax[1][rl]), fmaxf(wmax[2][rl], wmax[3][rl]));
+    // codes: thread t takes row t / 4, k (t % 4) * 32 .. + 31, so four
+    // consecutive lanes write one row's 128 bytes
+    const int rq = tid >> 2, seg = tid & 3;
+    const int64_t r = r0 + rq;
+    const float amax = fmaxf(fmaxf(wmax[0][rq], wmax[1][rq]), fmaxf(wmax[2][rq], wmax[3][rq]));
     const int e = block_exp(amax);
     u32x4 out[2];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-        const f4v x = {T[rl][w * 32 + 4 * j], T[rl][w * 32 + 4 * j + 1], T[rl][w * 32 + 4 * j + 2],
-                       T[rl][w * 32 + 4 * j + 3]};
+        const f4v x = {T[rq][seg * 32 + 4 * j], T[rq][seg * 32 + 4 * j + 1], T[rq][seg * 32 + 4 * j + 2],
+                       T[rq][seg * 32 + 4 * j + 3]};
         out[j / 4][j % 4] = pack4(x, e);
     }
-    u32x4 *dst = reinterpret_cast<u32x4 *>(Q + r * (KB * kQB) + k0 + w * 32);
+    u32x4 *dst = reinterpret_cast<u32x4 *>(Q + r * (KB * kQB) + k0 + seg * 32);
     dst[0] = out[0];
     dst[1] = out[1];
-    if (w == 0) S[scale_byte(r, kb, Rp)] = (uint8_t)(127 + e);
+    if (seg == 0) S[scale_byte(r, kb, Rp)] = (uint8_t)(127 + e);
 }
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
