@@ -466,7 +466,8 @@ MFMA_FORMS = [("bf16 16x16x32", 32 * 16 * 16 * 32 * 2, 2500.0, 4000),
               ("f32 16x16x4", 32 * 16 * 16 * 4 * 2, 157.3, 1000),
               ("f32 32x32x2", 8 * 32 * 32 * 2 * 2, 157.3, 1000),
               ("f64 16x16x4", 16 * 16 * 16 * 4 * 2, 78.6, 1000),
-              ("fp8 16x16x128 scaled", 16 * 16 * 16 * 128 * 2, 5000.0, 4000)]
+              ("fp8 16x16x128 scaled", 16 * 16 * 16 * 128 * 2, 5000.0, 4000),
+              ("fp8 32x32x64 scaled", 8 * 32 * 32 * 64 * 2, 5000.0, 4000)]
 
 
 def bench_mfma(args):

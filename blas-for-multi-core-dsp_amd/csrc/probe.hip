@@ -54,9 +54,10 @@ constexpr int kProbeVec = 1024;  // 16-byte operand words read by the MFMA probe
 // One wave per SIMD (4 per workgroup): each wave loads its operands once from
 // `src` (kProbeVec 16-byte words), then issues `iters` rounds of independent
 // MFMAs of one form on them (32 accumulator tiles for the 16 x 16 forms, 8 for
-// the 32 x 32 forms, 16 for FP64 and FP8), and writes one sum per thread.
-// Forms: 0 bf16 16x16x32, 1 bf16 32x32x16, 2 f16 16x16x32, 3 f32 16x16x4,
-// 4 f32 32x32x2, 5 f64 16x16x4, 6 fp8 16x16x128 with unit scales.
+// the 32 x 32 forms, 16 for FP64 and FP8 16 x 16), and writes one sum per
+// thread.  Forms: 0 bf16 16x16x32, 1 bf16 32x32x16, 2 f16 16x16x32, 3 f32
+// 16x16x4, 4 f32 32x32x2, 5 f64 16x16x4, 6 fp8 16x16x128 and 7 fp8 32x32x64,
+// both with unit scales.
 template <int FORM>
 __global__ void __launch_bounds__(256) mfma_probe_kernel(const u32x4 *__restrict__ src, float *__restrict__ out,
                                                          int iters)
@@ -153,6 +154,30 @@ __global__ void __launch_bounds__(256) mfma_probe_kernel(const u32x4 *__restrict
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j) s += (float)(acc[i][j][0] + acc[i][j][3]);
+    } else if constexpr (FORM == 7) {
+        i32x8 a[2], b[4];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const u32x4 x = word(i), y = word(i + 4);
+            a[i] = i32x8{(int)x[0], (int)x[1], (int)x[2], (int)x[3], (int)y[0], (int)y[1], (int)y[2], (int)y[3]};
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const u32x4 x = word(8 + j), y = word(2 + j);
+            b[j] = i32x8{(int)x[0], (int)x[1], (int)x[2], (int)x[3], (int)y[0], (int)y[1], (int)y[2], (int)y[3]};
+        }
+        f16v acc[2][4] = {};
+        for (int it = 0; it < iters; ++it)
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a[i], b[j], acc[i][j], 0, 0, 0, 127, 0,
+                                                                                127);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) s += acc[i][j][0] + acc[i][j][15];
     } else {
         i32x8 a[4], b[4];
 #pragma unroll
@@ -253,6 +278,7 @@ extern "C" int blasdsp_probe_mfma(int form, const void *src, void *out, int bloc
     case 4: hipLaunchKernelGGL(mfma_probe_kernel<4>, dim3(blocks), dim3(256), 0, s, in, o, iters); break;
     case 5: hipLaunchKernelGGL(mfma_probe_kernel<5>, dim3(blocks), dim3(256), 0, s, in, o, iters); break;
     case 6: hipLaunchKernelGGL(mfma_probe_kernel<6>, dim3(blocks), dim3(256), 0, s, in, o, iters); break;
+    case 7: hipLaunchKernelGGL(mfma_probe_kernel<7>, dim3(blocks), dim3(256), 0, s, in, o, iters); break;
     default: return BLASDSP_ERR_INVALID;
     }
     return hip_status(hipGetLastError());

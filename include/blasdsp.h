@@ -496,7 +496,8 @@ BLASDSP_API int blasdsp_probe_read(const void *p, int64_t bytes, void *out, int 
  * 256 * blocks floats.  Per wave and round: form 0 BF16 16x16x32 and 2 FP16
  * 16x16x32, 32 MFMAs; 1 BF16 32x32x16 and 4 FP32 32x32x2, 8 MFMAs; 3 FP32
  * 16x16x4, 32 MFMAs; 5 FP64 16x16x4 and 6 scaled FP8 16x16x128 (unit
- * scales), 16 MFMAs.  The caller times the launch. */
+ * scales), 16 MFMAs; 7 scaled FP8 32x32x64 (unit scales), 8 MFMAs.  The
+ * caller times the launch. */
 BLASDSP_API int blasdsp_probe_mfma(int form, const void *src, void *out, int blocks, int iters, void *stream);
 
 /* Global -> LDS operand-load throughput from an L2-resident region: `blocks`
