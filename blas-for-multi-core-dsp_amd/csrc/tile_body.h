@@ -4,8 +4,8 @@
 // A workgroup of 4 or 8 waves computes one BM x BM tile of C = alpha op(A)
 // op(B) + beta C with v_mfma_f64_16x16x4_f64 (FP64) or v_mfma_f32_16x16x4_f32
 // (FP32):
-//   * the waves form a 2 x 2 (or 4 x 2) grid, each owning a sub-tile of MFMA
-//     16x16 accumulators;
+//   * the waves form a 2 x 2 (or, with 8 waves, 4 x 2; 2 x 4 for TT) grid,
+//     each owning a sub-tile of MFMA 16x16 accumulators;
 //   * op(A) / op(B) tiles of BK = 128 bytes of K are staged global -> LDS with
 //     16-byte vector loads along whichever dimension is contiguous in HBM, and
 //     the LDS copy keeps that orientation; the MFMA fragment reads adapt to it,
@@ -64,20 +64,22 @@ struct Mma<float> {
     static __device__ __forceinline__ int out_row(int g, int r) { return 4 * g + r; }
 };
 
-// Tile geometry.  NW waves (4 or 8) in a WGM x 2 grid; each owns a
-// WM x WN sub-tile of MI x NJ MFMA 16x16 accumulators.
-template <typename T, int BM_, int NW_ = 4>
+// Tile geometry.  NW waves (4 or 8) in a WGM x WGN grid (WGN 2, or 4 for
+// TT with 8 waves); each owns a WM x WN sub-tile of MI x NJ MFMA 16x16
+// accumulators.
+template <typename T, int BM_, int NW_ = 4, int WGN_ = 2>
 struct Cfg {
     static constexpr int BM = BM_;
     static constexpr int NW = NW_;
     static_assert(NW == 4 || NW == 8, "4 or 8 waves");
+    static_assert(NW % WGN_ == 0 && (WGN_ == 2 || WGN_ == 4), "wave grid");
     static constexpr int BK = 128 / (int)sizeof(T);  // one 128-byte row of K
     static constexpr int V = 16 / (int)sizeof(T);    // elements per 16-byte vector
     static constexpr int SK = BK + V;                // k-contiguous LDS row stride
     static constexpr int SMN = BM + 16;              // m/n-contiguous LDS row stride
     static constexpr int BUF = (BM * SK > BK * SMN) ? BM * SK : BK * SMN;  // per operand stage
     static constexpr int THREADS = 64 * NW;
-    static constexpr int WGM = NW / 2, WGN = 2;      // wave grid
+    static constexpr int WGM = NW / WGN_, WGN = WGN_;  // wave grid
     static constexpr int WM = BM / WGM, WN = BM / WGN;
     static constexpr int MI = WM / 16, NJ = WN / 16;
     static constexpr int LDS_ELEMS = 2 * 2 * BUF;    // [stage][A, B]
@@ -271,13 +273,22 @@ constexpr int64_t kPrefetchCK = 2048;
 // split-K partial.  The MFMA sequence of every C element is the same for any
 // NW, FULL and PTR, so those variants give bit-identical results.  tid: the
 // thread's index in the workgroup (threadIdx.x when negative).
+// Wave grid of the 8-wave tiles: 4 x 2 (each wave 32 rows x 64 columns, so
+// it reads 2 op(A) and 4 op(B) fragment tiles per k step), except TT, which
+// takes 2 x 4.  A K-contiguous operand's LDS rows have 2-way bank conflicts
+// (op(A) in NN and NT, op(B) in NT and TT); TT's conflicting operand is then
+// read half as often.  Measured on MI355X (bit-identical): DGEMM 4096^3 TT
+// 67.8 -> 69.6 TFLOP/s, FP32 8192^3 TT +0.7 %; NN with 2 x 4 reads its
+// conflicting op(A) twice as often, and NT has both operands K-contiguous.
+constexpr bool kFlipTT = true;
+
 template <typename T, int BM, bool TA, bool TB, int NW = 4, bool FULL = false, bool PTR = false>
 __device__ __forceinline__ void tile_body(const Gemm<T> &g, int64_t m0, int64_t n0, T *lds, int64_t kb = 0,
                                           int64_t ke = -1, T *W = nullptr, int tid = -1)
 {
     if (ke < 0) ke = g.K;
     if (tid < 0) tid = threadIdx.x;
-    using Cf = Cfg<T, BM, NW>;
+    using Cf = Cfg<T, BM, NW, (kFlipTT && NW == 8 && TA && TB) ? 4 : 2>;
     using acc_t = typename Mma<T>::acc_t;
     constexpr int BK = Cf::BK, SK = Cf::SK, SMN = Cf::SMN, MI = Cf::MI, NJ = Cf::NJ, BUF = Cf::BUF;
 
