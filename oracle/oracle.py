@@ -223,7 +223,8 @@ def projected_errors(G, RX, X, side: str = "right") -> tuple[float, float, int]:
     den = np.linalg.norm(RX)
     nw = float(np.linalg.norm(E) / den) if den > 0 else (0.0 if not E.any() else np.inf)
     g = np.sqrt(np.einsum("ij,ij->i" if right else "ij,ij->j", G, G, dtype=np.float64))
-    line = np.linalg.norm(E, axis=1) / (np.sqrt(S) * np.maximum(g, np.finfo(np.float64).tiny))
+    with np.errstate(invalid="ignore", divide="ignore"):  # a non-finite G gives non-finite estimates
+        line = np.linalg.norm(E, axis=1) / (np.sqrt(S) * np.maximum(g, np.finfo(np.float64).tiny))
     i = int(np.argmax(line))
     return nw, float(line[i]), i
 

@@ -351,6 +351,17 @@ def test_projected_errors_find_one_wrong_element(orc):
     assert j == j0 and col > 2e-10, (j, col)
 
 
+def test_projected_errors_fail_on_nan_and_inf(orc):
+    """A NaN or an infinity anywhere in the result makes both estimates
+    non-finite, so every bar written as `estimate <= bar` fails."""
+    A, B, C0, R, X, RX, _ = _projection_case(orc, 60, 50, 40)
+    for bad in (np.nan, np.inf):
+        G = R.copy()
+        G[37, 11] = bad
+        nw, row, i = orc.projected_errors(G, RX, X)
+        assert not nw <= 1.0 and not row <= 1.0, (bad, nw, row)
+
+
 def test_gemm_projection_modes_and_f32(orc):
     """Every storage mode of A and B gives the same R X; FP32 operands
     project exactly as their FP64 values do."""
