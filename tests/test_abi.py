@@ -223,3 +223,24 @@ def test_noalloc_rejects_oversized_explicit_split(blas):
     assert L.blasdsp_gemm_noalloc(ctypes.byref(bad), blas.ALGO_TILE, 1, None) == blas.ERR_INVALID
     short = blas.GemmDesc(blas.FLOAT32, b"N", b"N", n, n, n, 1.0, 0.0, FAKE, n - 1, FAKE, n, FAKE, n)
     assert L.blasdsp_gemm_noalloc(ctypes.byref(short), blas.ALGO_AUTO, 0, None) == blas.ERR_INVALID
+
+
+def test_probe_arguments_rejected(blas):
+    """The measurement probes refuse unknown forms and modes, null or
+    misaligned buffers and empty grids before any launch."""
+    L = blas.lib()
+    bad_src = ctypes.c_void_p(0x1004)  # not 16-byte aligned
+    for form in (-1, 8):
+        assert L.blasdsp_probe_mfma(form, FAKE, FAKE, 1, 1, None) == blas.ERR_INVALID, form
+    assert L.blasdsp_probe_mfma(0, None, FAKE, 1, 1, None) == blas.ERR_INVALID
+    assert L.blasdsp_probe_mfma(0, bad_src, FAKE, 1, 1, None) == blas.ERR_INVALID
+    assert L.blasdsp_probe_mfma(7, FAKE, None, 1, 1, None) == blas.ERR_INVALID
+    assert L.blasdsp_probe_mfma(7, FAKE, FAKE, 0, 1, None) == blas.ERR_INVALID
+    assert L.blasdsp_probe_mfma(7, FAKE, FAKE, 1, 0, None) == blas.ERR_INVALID
+    for mode in (-1, 3):
+        assert L.blasdsp_probe_lds(mode, FAKE, FAKE, 1, 1, None) == blas.ERR_INVALID, mode
+    assert L.blasdsp_probe_lds(0, None, FAKE, 1, 1, None) == blas.ERR_INVALID
+    assert L.blasdsp_probe_lds(0, FAKE, FAKE, 1, 0, None) == blas.ERR_INVALID
+    for ptr, nbytes in ((FAKE, 8), (FAKE, 24), (bad_src, 64), (None, 64)):
+        assert L.blasdsp_probe_read(ptr, nbytes, FAKE, 1, None) == blas.ERR_INVALID, (ptr, nbytes)
+    assert L.blasdsp_probe_read(FAKE, 64, FAKE, 0, None) == blas.ERR_INVALID
